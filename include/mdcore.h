@@ -1,0 +1,420 @@
+/*
+ * mdcore host API for the MI355X nonbonded engine.
+ *
+ * This header keeps the C API of mdcore's nonbonded path so that drivers
+ * written against the reference (examples/argon/argon.c, examples/bulk/bulk.c)
+ * compile against it: engine_init / engine_addtype / engine_addpot /
+ * engine_load / engine_start / engine_step / engine_unload, the
+ * potential_create_* piecewise-quintic tables, and the space/cell/task data
+ * model.  The host build is double precision (FPTYPE = double); the GPU
+ * computes pair arithmetic in FP32 (see engine_gpu.h).
+ *
+ * Reference interfaces mirrored here (file:line in the reference tree):
+ *   struct part            src/part.h:46-69
+ *   struct part_type       src/part.h:72-89
+ *   struct potential       src/potential.h:57-74, constants 31-40
+ *   struct cell            src/cell.h:58-118, flags 34-44
+ *   struct task            src/task.h:24-88
+ *   struct space           src/space.h:65-142, constants 31-51
+ *   struct engine          src/engine.h:179-300, error codes 21-52, flags 56-72
+ *   errs_register/dump     src/errs.h, src/errs.c:61-111
+ */
+#ifndef MDCORE_H
+#define MDCORE_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+/* Host floating-point type: the double build of the reference (fptype.h:21-40). */
+typedef double FPTYPE;
+#define FPTYPE_DOUBLE 1
+
+/* ------------------------------------------------------------------ errs */
+
+/* Error stack (errs.c:61-111): every failing call pushes one entry. */
+int errs_register(int id, const char *msg, int line, const char *func, const char *file);
+int errs_dump(FILE *out);
+int errs_clear(void);
+/* Copy the error stack as text into buf (for callers without a FILE*). */
+int errs_gettext(char *buf, int size);
+
+/* ------------------------------------------------------------------ part */
+
+#define part_flag_none   0
+#define part_flag_frozen 1
+#define part_flag_ghost  2
+
+/* Particle record (part.h:46-69): positions are relative to the owning cell. */
+struct part {
+    FPTYPE x[4] __attribute__((aligned(16)));
+    FPTYPE v[4] __attribute__((aligned(16)));
+    FPTYPE f[4] __attribute__((aligned(16)));
+    float q;
+    int id, vid;
+    short int type;
+    unsigned short int flags;
+} __attribute__((aligned(64)));
+
+struct part_type {
+    int id;
+    double mass, imass, charge;
+    double eps, rmin;
+    char name[64], name2[64];
+};
+
+/* ------------------------------------------------------------------ potential */
+
+#define potential_err_ok        0
+#define potential_err_null      -1
+#define potential_err_malloc    -2
+#define potential_err_bounds    -3
+#define potential_err_nyi       -4
+#define potential_err_ivalsmax  -5
+
+#define potential_degree   5
+#define potential_chunk    (potential_degree + 3)
+#define potential_ivalsa   1
+#define potential_ivalsb   10
+#define potential_N        100
+#define potential_align    64
+#define potential_ivalsmax 320
+
+/* Electrostatic prefactor 1/(4 pi) of the reference (potential.h:40): the
+   reference's Coulomb tables are in units where eps0 = 1. */
+#define potential_escale (0.079577471545947667882)
+
+#define potential_flag_none    0
+#define potential_flag_LJ126   1
+#define potential_flag_Ewald   2
+#define potential_flag_Coulomb 4
+
+extern int potential_err;
+
+/* A piecewise-quintic table on [a,b] with n intervals.  c holds (n+1) chunks
+   of potential_chunk coefficients {centre, 1/halfwidth, x^5..x^0}; chunk 0 is
+   the linear extension below a (potential.c:1167-1185). */
+struct potential {
+    FPTYPE alpha[4];
+    FPTYPE *c;
+    double a, b;
+    unsigned int flags;
+    int n;
+};
+
+extern struct potential potential_null;
+
+int potential_init(struct potential *p, double (*f)(double), double (*fp)(double),
+                   double (*f6p)(double), FPTYPE a, FPTYPE b, FPTYPE tol);
+int potential_getcoeffs(double (*f)(double), double (*fp)(double), FPTYPE *xi, int n,
+                        FPTYPE *c, FPTYPE *err);
+double potential_getalpha(double (*f6p)(double), double a, double b);
+void potential_clear(struct potential *p);
+
+struct potential *potential_create_LJ126(double a, double b, double A, double B, double tol);
+struct potential *potential_create_LJ126_switch(double a, double b, double A, double B, double s, double tol);
+struct potential *potential_create_LJ126_Ewald(double a, double b, double A, double B, double q, double kappa, double tol);
+struct potential *potential_create_LJ126_Ewald_switch(double a, double b, double A, double B, double q, double kappa, double s, double tol);
+struct potential *potential_create_LJ126_Coulomb(double a, double b, double A, double B, double q, double tol);
+struct potential *potential_create_Ewald(double a, double b, double q, double kappa, double tol);
+struct potential *potential_create_Coulomb(double a, double b, double q, double tol);
+struct potential *potential_create_harmonic(double a, double b, double K, double r0, double tol);
+
+/* New (no reference counterpart; built through the reference's extension point
+   potential_init(f, fp, f6p, ...), potential.h:83): Lennard-Jones 12-6 plus
+   reaction-field Coulomb, shifted so that E(rc) = 0:
+     E(r) = A/r^12 - B/r^6 + ke*q*(1/r + krf*r^2 - crf),
+     krf = (eps_rf - 1)/((2 eps_rf + 1) rc^3), crf = 1/rc + krf rc^2.
+   ke is the Coulomb constant in the caller's units (138.935458 kJ mol^-1 nm e^-2
+   for GROMACS units); eps_rf <= 0 means eps_rf = infinity. */
+struct potential *potential_create_LJ126_RF(double a, double b, double A, double B, double q,
+                                            double ke, double eps_rf, double tol);
+
+/* Host FP64 evaluation of a table (potential_eval.h:67-104): e and f=(dE/dr)/r. */
+void potential_eval(const struct potential *p, FPTYPE r2, FPTYPE *e, FPTYPE *f);
+
+/* Analytic helpers (potential.c:66-267). */
+double potential_LJ126(double r, double A, double B);
+double potential_LJ126_p(double r, double A, double B);
+double potential_LJ126_6p(double r, double A, double B);
+double potential_Coulomb(double r);
+double potential_Coulomb_p(double r);
+double potential_Coulomb_6p(double r);
+double potential_Ewald(double r, double kappa);
+double potential_Ewald_p(double r, double kappa);
+double potential_Ewald_6p(double r, double kappa);
+double potential_switch(double r, double A, double B);
+double potential_switch_p(double r, double A, double B);
+
+/* ------------------------------------------------------------------ cell */
+
+#define cell_err_ok      0
+#define cell_err_null    -1
+#define cell_err_malloc  -2
+#define cell_err_pthread -3
+
+#define cell_incr      10
+#define cell_partalign 64
+
+#define cell_flag_none   0
+#define cell_flag_ghost  1
+#define cell_flag_wait   2
+#define cell_flag_waited 4
+#define cell_flag_marked 8
+
+/* Direction tables (cell.c:56-101): sort-list id of each of the 27 neighbour
+   offsets, the 13 unit axes, and whether a pair must be flipped. */
+extern const char cell_sortlistID[27];
+extern const FPTYPE cell_shift[13 * 3];
+extern const char cell_flip[27];
+
+struct cell {
+    unsigned int flags;
+    int id;
+    int loc[3];
+    double origin[3];
+    double dim[3];
+    int size, count;
+    struct part *parts;
+    double epot;
+    struct part *incomming;
+    int incomming_size, incomming_count;
+    FPTYPE maxdx;
+    /* 13 sorted lists of (index << 16 | key), count entries each (cell.c:443). */
+    unsigned int *sortlist;
+    /* Bitmask of the sort directions this cell's pair tasks need (space.c:745-752). */
+    unsigned int sortflags;
+};
+
+int cell_init(struct cell *c, int *loc, double *origin, double *dim);
+struct part *cell_add(struct cell *c, struct part *p, struct part **partlist);
+struct part *cell_add_incomming(struct cell *c, struct part *p);
+int cell_welcome(struct cell *c, struct part **partlist);
+int cell_flush(struct cell *c, struct part **partlist, struct cell **celllist);
+
+/* ------------------------------------------------------------------ task */
+
+#define task_type_none 0
+#define task_type_self 1
+#define task_type_pair 2
+#define task_type_sort 3
+
+#define task_subtype_none 0
+
+/* Cell-pair / self / sort task (task.h:24-88).  For pairs, flags = sid. */
+struct task {
+    int type, subtype, flags;
+    int i, j;
+    int wait;
+};
+
+/* ------------------------------------------------------------------ space */
+
+#define space_err_ok       0
+#define space_err_null     -1
+#define space_err_malloc   -2
+#define space_err_cell     -3
+#define space_err_pthread  -4
+#define space_err_range    -5
+#define space_err_maxpairs -6
+#define space_err_nrtasks  -7
+#define space_err_task     -8
+
+#define space_periodic_none       0
+#define space_periodic_x          1
+#define space_periodic_y          2
+#define space_periodic_z          4
+#define space_periodic_full       7
+#define space_periodic_ghost_x    8
+#define space_periodic_ghost_y    16
+#define space_periodic_ghost_z    32
+#define space_periodic_ghost_full 56
+
+#define space_partlist_incr 100
+
+#define space_cellid(s, i, j, k) (((i) * (s)->cdim[1] + (j)) * (s)->cdim[2] + (k))
+
+extern int space_err;
+
+struct space {
+    double dim[3];
+    double origin[3];
+    int cdim[3];
+    int span[3];
+    double h[3], ih[3];
+    double cutoff, cutoff2;
+    unsigned int period;
+    int nr_cells;
+    int *cid_real, *cid_ghost, *cid_marked;
+    int nr_real, nr_ghost, nr_marked;
+    struct cell *cells;
+    int nr_tasks, tasks_size;
+    struct task *tasks;
+    struct part **partlist;
+    struct cell **celllist;
+    int nr_parts, size_parts;
+    int verlet_rebuild;
+    FPTYPE maxdx;
+    double epot, epot_nonbond, epot_bond, epot_angle, epot_dihedral, epot_exclusion;
+    /* Scalar virial sum_pairs dx.F of the last nonbonded pass (new). */
+    double virial;
+    /* Pair interactions evaluated by the last nonbonded pass (new). */
+    long long nr_pairs;
+    /* Exclusions subtracted by the last pass (new). */
+    long long nr_excl_evaluated;
+};
+
+int space_init(struct space *s, const double *origin, const double *dim, double *L, double cutoff,
+               unsigned int period);
+int space_getsid(struct space *s, struct cell **ci, struct cell **cj, FPTYPE *shift);
+int space_addpart(struct space *s, struct part *p, double *x);
+int space_prepare(struct space *s);
+int space_shuffle_local(struct space *s);
+int space_getpos(struct space *s, int id, double *x);
+int space_flush(struct space *s);
+void space_free(struct space *s);
+
+/* ------------------------------------------------------------------ exclusion */
+
+struct exclusion {
+    int i, j;
+};
+
+/* ------------------------------------------------------------------ engine */
+
+#define engine_err_ok        0
+#define engine_err_null      -1
+#define engine_err_malloc    -2
+#define engine_err_space     -3
+#define engine_err_pthread   -4
+#define engine_err_runner    -5
+#define engine_err_range     -6
+#define engine_err_cell      -7
+#define engine_err_domain    -8
+#define engine_err_nompi     -9
+#define engine_err_mpi       -10
+#define engine_err_bond      -11
+#define engine_err_angle     -12
+#define engine_err_reader    -13
+#define engine_err_psf       -14
+#define engine_err_pdb       -15
+#define engine_err_cpf       -16
+#define engine_err_potential -17
+#define engine_err_exclusion -18
+#define engine_err_sets      -19
+#define engine_err_dihedral  -20
+/* engine_err_cuda (-21) of the reference: any HIP runtime failure. */
+#define engine_err_gpu       -21
+#define engine_err_nogpu     -22
+#define engine_err_cudasp    -23
+#define engine_err_maxparts  -24
+#define engine_err_queue     -25
+#define engine_err_rigid     -26
+#define engine_err_cutoff    -27
+#define engine_err_nometis   -28
+#define engine_err_deps      -29
+#define engine_err_nyi       -30
+
+#define engine_flag_none          0
+#define engine_flag_static        1
+#define engine_flag_localparts    2
+/* Bit 4 is the reference's engine_flag_cuda: here it selects the GPU path. */
+#define engine_flag_gpu           4
+#define engine_flag_cuda          engine_flag_gpu
+#define engine_flag_explepot      8
+#define engine_flag_verlet        16
+#define engine_flag_verlet_pairwise 32
+#define engine_flag_affinity      64
+#define engine_flag_prefetch      128
+#define engine_flag_verlet_pseudo 256
+#define engine_flag_unsorted      512
+#define engine_flag_shake         1024
+#define engine_flag_mpi           2048
+#define engine_flag_parbonded     4096
+#define engine_flag_async         8192
+#define engine_flag_dpd           16384
+#define engine_flag_nullpart      32768
+/* New: evaluate the sorted window with the projected shift shift.n instead of
+   the reference's |shift| (runner_dopair.c:451-452).  The projected window
+   never drops an in-cutoff pair; the reference window does for cell offsets
+   with more than one non-zero component and span > 1. */
+#define engine_flag_projwindow    65536
+
+extern int engine_err;
+extern const char *engine_err_msg[];
+
+struct engine {
+    unsigned int flags;
+    struct space s;
+    int time;
+    double dt;
+    int max_type;
+    int nr_types;
+    struct part_type *types;
+    /* Type-pair potentials, max_type x max_type, NULL = no interaction (engine.h:192). */
+    struct potential **p;
+    struct exclusion *exclusions;
+    int nr_exclusions, exclusions_size;
+    int nr_runners;
+    /* Opaque device state owned by the GPU layer (engine_gpu.h). */
+    void *gpu;
+    /* Wall-clock seconds of the last engine_step phases (new). */
+    double timers[8];
+};
+
+enum {
+    engine_timer_step = 0,
+    engine_timer_prepare,
+    engine_timer_nonbond,
+    engine_timer_advance,
+    engine_timer_shuffle,
+    engine_timer_gpu_load,
+    engine_timer_gpu_kernels,
+    engine_timer_gpu_unload
+};
+
+int engine_init(struct engine *e, const double *origin, const double *dim, double *L, double cutoff,
+                unsigned int period, int max_type, unsigned int flags);
+int engine_addtype(struct engine *e, double mass, double charge, char *name, char *name2);
+int engine_addpot(struct engine *e, struct potential *p, int i, int j);
+int engine_load(struct engine *e, double *x, double *v, int *type, int *pid, int *vid, double *charge,
+                unsigned int *flags, int N);
+int engine_unload(struct engine *e, double *x, double *v, int *type, int *pid, int *vid, double *charge,
+                  unsigned int *flags, double *epot, int N);
+int engine_start(struct engine *e, int nr_runners, int nr_queues);
+int engine_step(struct engine *e);
+int engine_nonbond_eval(struct engine *e);
+int engine_advance(struct engine *e);
+int engine_shuffle(struct engine *e);
+int engine_exclusion_add(struct engine *e, int i, int j);
+int engine_exclusion_shrink(struct engine *e);
+int engine_finalize(struct engine *e);
+
+/* Helpers for callers that cannot reach into struct engine (ctypes, other
+   FFIs).  New; the reference's drivers read e.s.cells[].parts[] directly. */
+unsigned long engine_sizeof(void);
+struct engine *engine_alloc(void);
+void engine_free(struct engine *e);
+/* Per-particle forces f[3*id+k] (host copy), by particle id. */
+int engine_getforces(struct engine *e, double *f, int N);
+/* Absolute positions x[3*id+k], by particle id. */
+int engine_getpositions(struct engine *e, double *x, int N);
+/* energies[0..3] = epot, epot_nonbond, epot_exclusion, virial; counts[0..1] = pairs, exclusions. */
+int engine_getenergies(struct engine *e, double *energies, long long *counts);
+/* Copy the host sortlist of cell cid, direction sid, into out (count entries). */
+int engine_getsortlist(struct engine *e, int cid, int sid, unsigned int *out);
+/* Per-cell geometry: counts[cid], and the particle ids in cell order (N entries). */
+int engine_getcells(struct engine *e, int *counts, int *ids);
+void potential_free(struct potential *p);
+/* Read back a table: n, a, b, alpha[4], c[(n+1)*8]. */
+int potential_getdata(const struct potential *p, int *n, double *ab, double *alpha, double *c, int csize);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDCORE_H */

@@ -1,0 +1,354 @@
+"""ctypes mirror of the mdcore C API for the MI355X nonbonded engine.
+
+The C library (lib/libmdcore_gpu.so, built by the Makefile next to this file)
+keeps the reference's host API -- engine_init / engine_addtype /
+engine_addpot / engine_load / engine_start / engine_step / engine_unload and
+the potential_create_* tables (reference src/engine.h:319-368,
+src/potential.h:82-95) -- and the C-ABI of the GPU path
+(include/engine_gpu.h, replacing engine_cuda_* of src/engine.h:381-398).
+This module only declares those symbols for Python callers; every
+computation happens in the library.  There is no Python or CPU fallback:
+loading fails loudly if the library is missing.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBPATH = HERE / "lib" / "libmdcore_gpu.so"
+
+# engine flags (include/mdcore.h, reference src/engine.h:56-72)
+engine_flag_none = 0
+engine_flag_gpu = 4
+engine_flag_cuda = 4
+engine_flag_nullpart = 32768
+engine_flag_projwindow = 65536
+
+space_periodic_full = 7
+
+engine_gpu_sync_parts_up = 1
+engine_gpu_sync_forces_down = 2
+engine_gpu_sync_sort_down = 4
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_uint)
+_llp = C.POINTER(C.c_longlong)
+
+_lib = None
+
+
+class MdcoreError(RuntimeError):
+    pass
+
+
+def library():
+    """Load lib/libmdcore_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBPATH.exists():
+        raise MdcoreError(f"{LIBPATH} is missing: build it with `make -C {HERE}`")
+    lib = C.CDLL(str(LIBPATH), mode=C.RTLD_GLOBAL)
+    sig = {
+        "errs_gettext": (C.c_int, [C.c_char_p, C.c_int]),
+        "errs_clear": (C.c_int, []),
+        "engine_alloc": (C.c_void_p, []),
+        "engine_free": (None, [C.c_void_p]),
+        "engine_sizeof": (C.c_ulong, []),
+        "engine_init": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_uint, C.c_int, C.c_uint]),
+        "engine_addtype": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_char_p, C.c_char_p]),
+        "engine_addpot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+        "engine_load": (C.c_int, [C.c_void_p, _dp, _dp, _ip, _ip, _ip, _dp, _up, C.c_int]),
+        "engine_unload": (C.c_int, [C.c_void_p, _dp, _dp, _ip, _ip, _ip, _dp, _up, _dp, C.c_int]),
+        "engine_start": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "engine_step": (C.c_int, [C.c_void_p]),
+        "engine_shuffle": (C.c_int, [C.c_void_p]),
+        "engine_nonbond_eval": (C.c_int, [C.c_void_p]),
+        "engine_exclusion_add": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "engine_exclusion_shrink": (C.c_int, [C.c_void_p]),
+        "engine_finalize": (C.c_int, [C.c_void_p]),
+        "engine_getforces": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+        "engine_getpositions": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+        "engine_getenergies": (C.c_int, [C.c_void_p, _dp, _llp]),
+        "engine_getsortlist": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up]),
+        "engine_getcells": (C.c_int, [C.c_void_p, _ip, _ip]),
+        "space_prepare": (C.c_int, [C.c_void_p]),
+        "engine_gpu_setdevice": (C.c_int, [C.c_void_p, C.c_int]),
+        "engine_gpu_load": (C.c_int, [C.c_void_p]),
+        "engine_nonbond_gpu": (C.c_int, [C.c_void_p]),
+        "engine_gpu_sync": (C.c_int, [C.c_void_p, C.c_uint]),
+        "engine_gpu_finalize": (C.c_int, [C.c_void_p]),
+        "engine_gpu_time": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+        "engine_gpu_results": (C.c_int, [C.c_void_p, _dp, _llp]),
+        "potential_create_LJ126": (C.c_void_p, [C.c_double] * 5),
+        "potential_create_LJ126_switch": (C.c_void_p, [C.c_double] * 6),
+        "potential_create_LJ126_Coulomb": (C.c_void_p, [C.c_double] * 6),
+        "potential_create_LJ126_Ewald": (C.c_void_p, [C.c_double] * 7),
+        "potential_create_Ewald": (C.c_void_p, [C.c_double] * 5),
+        "potential_create_Coulomb": (C.c_void_p, [C.c_double] * 4),
+        "potential_create_harmonic": (C.c_void_p, [C.c_double] * 5),
+        "potential_create_LJ126_RF": (C.c_void_p, [C.c_double] * 8),
+        "potential_getdata": (C.c_int, [C.c_void_p, _ip, _dp, _dp, _dp, C.c_int]),
+        "potential_eval": (None, [C.c_void_p, C.c_double, _dp, _dp]),
+        "potential_free": (None, [C.c_void_p]),
+        "md_gen_fcc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp]),
+        "md_gen_maxwell": (C.c_int, [C.c_int, _dp, C.c_double, C.c_uint64, _dp]),
+        "md_gen_water": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_uint64, _dp]),
+        "md_gen_rsa": (C.c_int, [C.c_int, _dp, C.c_double, C.c_uint64, C.c_longlong, _dp]),
+        "md_gen_types": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, _ip]),
+        "md_gen_uniform": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def errors():
+    buf = C.create_string_buffer(8192)
+    library().errs_gettext(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what):
+    if rc is None or (isinstance(rc, int) and rc < 0):
+        msg = errors()
+        library().errs_clear()
+        raise MdcoreError(f"{what} failed ({rc}):\n{msg}")
+    return rc
+
+
+def ptr(a, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+# ---------------------------------------------------------------- potentials
+
+class Potential:
+    """A host table built by one of the potential_create_* constructors."""
+
+    def __init__(self, handle, desc):
+        if not handle:
+            msg = errors()
+            library().errs_clear()
+            raise MdcoreError(f"potential {desc} failed:\n{msg}")
+        self.handle = C.c_void_p(handle)
+        self.desc = desc
+
+    def data(self):
+        lib = library()
+        n = C.c_int(0)
+        ab = np.zeros(2)
+        alpha = np.zeros(4)
+        check(lib.potential_getdata(self.handle, C.byref(n), ptr(ab), ptr(alpha), None, 0), "potential_getdata")
+        c = np.zeros((n.value + 1) * 8)
+        check(lib.potential_getdata(self.handle, None, None, None, ptr(c), len(c)), "potential_getdata")
+        return {"n": n.value, "a": ab[0], "b": ab[1], "alpha": alpha, "c": c}
+
+    def eval(self, r2):
+        e = C.c_double(0.0)
+        f = C.c_double(0.0)
+        library().potential_eval(self.handle, float(r2), C.byref(e), C.byref(f))
+        return e.value, f.value
+
+
+def LJ126(a, b, A, B, tol):
+    return Potential(library().potential_create_LJ126(a, b, A, B, tol), f"LJ126({a},{b},{A},{B},{tol})")
+
+
+def LJ126_switch(a, b, A, B, s, tol):
+    return Potential(library().potential_create_LJ126_switch(a, b, A, B, s, tol), "LJ126_switch")
+
+
+def LJ126_Coulomb(a, b, A, B, q, tol):
+    return Potential(library().potential_create_LJ126_Coulomb(a, b, A, B, q, tol), "LJ126_Coulomb")
+
+
+def LJ126_Ewald(a, b, A, B, q, kappa, tol):
+    return Potential(library().potential_create_LJ126_Ewald(a, b, A, B, q, kappa, tol), "LJ126_Ewald")
+
+
+def Ewald(a, b, q, kappa, tol):
+    return Potential(library().potential_create_Ewald(a, b, q, kappa, tol), "Ewald")
+
+
+def Coulomb(a, b, q, tol):
+    return Potential(library().potential_create_Coulomb(a, b, q, tol), "Coulomb")
+
+
+def harmonic(a, b, K, r0, tol):
+    return Potential(library().potential_create_harmonic(a, b, K, r0, tol), "harmonic")
+
+
+def LJ126_RF(a, b, A, B, q, ke, eps_rf, tol):
+    return Potential(library().potential_create_LJ126_RF(a, b, A, B, q, ke, eps_rf, tol),
+                     f"LJ126_RF({a},{b},{A},{B},{q},{ke},{eps_rf},{tol})")
+
+
+# ---------------------------------------------------------------- engine
+
+class Engine:
+    """One struct engine (engine_alloc + engine_init)."""
+
+    def __init__(self, origin, dim, L, cutoff, max_type, flags=engine_flag_gpu, period=space_periodic_full):
+        lib = library()
+        self.lib = lib
+        self.e = C.c_void_p(lib.engine_alloc())
+        if not self.e:
+            raise MdcoreError("engine_alloc failed")
+        o = np.asarray(origin, dtype=np.float64)
+        d = np.asarray(dim, dtype=np.float64)
+        l = np.asarray(L, dtype=np.float64)
+        check(lib.engine_init(self.e, ptr(o), ptr(d), ptr(l), float(cutoff), period, int(max_type), flags),
+              "engine_init")
+        self.flags = flags
+        self.N = 0
+        self._pots = []
+
+    def addtype(self, mass, charge=0.0, name="X"):
+        return check(self.lib.engine_addtype(self.e, mass, charge, name.encode(), name.encode()), "engine_addtype")
+
+    def addpot(self, pot, i, j):
+        self._pots.append(pot)
+        check(self.lib.engine_addpot(self.e, pot.handle, i, j), "engine_addpot")
+
+    def load(self, x, types, v=None, ids=None):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        N = len(t)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        ii = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        check(self.lib.engine_load(self.e, ptr(x), ptr(vv), ptr(t, C.c_int), ptr(ii, C.c_int), None, None, None, N),
+              "engine_load")
+        self.N += N
+
+    def exclusion_add(self, i, j):
+        check(self.lib.engine_exclusion_add(self.e, int(i), int(j)), "engine_exclusion_add")
+
+    def exclusions_add(self, pairs):
+        for i, j in np.asarray(pairs, dtype=np.int64).reshape(-1, 2):
+            check(self.lib.engine_exclusion_add(self.e, int(i), int(j)), "engine_exclusion_add")
+
+    def exclusion_shrink(self):
+        check(self.lib.engine_exclusion_shrink(self.e), "engine_exclusion_shrink")
+
+    def start(self, nr_runners=1):
+        check(self.lib.engine_start(self.e, nr_runners, nr_runners), "engine_start")
+
+    def step(self):
+        check(self.lib.engine_step(self.e), "engine_step")
+
+    def shuffle(self):
+        check(self.lib.engine_shuffle(self.e), "engine_shuffle")
+
+    def prepare(self):
+        check(self.lib.space_prepare(self.e), "space_prepare")
+
+    def nonbond(self):
+        """One nonbonded pass (space_prepare + engine_nonbond_gpu), no integration."""
+        self.prepare()
+        check(self.lib.engine_nonbond_eval(self.e), "engine_nonbond_eval")
+
+    def forces(self):
+        f = np.zeros(3 * self.N)
+        check(self.lib.engine_getforces(self.e, ptr(f), self.N), "engine_getforces")
+        return f.reshape(-1, 3)
+
+    def positions(self):
+        x = np.zeros(3 * self.N)
+        check(self.lib.engine_getpositions(self.e, ptr(x), self.N), "engine_getpositions")
+        return x.reshape(-1, 3)
+
+    def energies(self):
+        en = np.zeros(4)
+        cnt = np.zeros(2, dtype=np.int64)
+        check(self.lib.engine_getenergies(self.e, ptr(en), ptr(cnt, C.c_longlong)), "engine_getenergies")
+        return {"epot": en[0], "epot_nonbond": en[1], "epot_exclusion": en[2], "virial": en[3],
+                "pairs": int(cnt[0]), "exclusions": int(cnt[1])}
+
+    def gpu_results(self):
+        out = np.zeros(3)
+        cnt = np.zeros(3, dtype=np.int64)
+        check(self.lib.engine_gpu_results(self.e, ptr(out), ptr(cnt, C.c_longlong)), "engine_gpu_results")
+        return {"epot": out[0], "epot_exclusion": out[1], "virial": out[2], "pairs": int(cnt[0]),
+                "exclusions": int(cnt[1]), "pairs_refwindow": int(cnt[2])}
+
+    def sync(self, what):
+        check(self.lib.engine_gpu_sync(self.e, what), "engine_gpu_sync")
+
+    def gpu_time(self, steps, warmup):
+        ms = np.zeros(4)
+        check(self.lib.engine_gpu_time(self.e, steps, warmup, ptr(ms)), "engine_gpu_time")
+        return {"sort_ms": ms[0], "nonbond_ms": ms[1], "reduce_ms": ms[2], "total_ms": ms[3]}
+
+    def cells(self, ncells):
+        counts = np.zeros(ncells, dtype=np.int32)
+        ids = np.zeros(max(self.N, 1), dtype=np.int32)
+        n = check(self.lib.engine_getcells(self.e, ptr(counts, C.c_int), ptr(ids, C.c_int)), "engine_getcells")
+        return counts, ids[:n]
+
+    def sortlist(self, cid, sid, count):
+        out = np.zeros(max(count, 1), dtype=np.uint32)
+        check(self.lib.engine_getsortlist(self.e, cid, sid, ptr(out, C.c_uint)), "engine_getsortlist")
+        return out[:count]
+
+    def close(self):
+        if self.e:
+            self.lib.engine_free(self.e)
+            self.e = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- workloads
+
+def gen_fcc(nx, ny, nz, a, offset):
+    x = np.zeros(4 * nx * ny * nz * 3)
+    check(library().md_gen_fcc(nx, ny, nz, a, offset, ptr(x)), "md_gen_fcc")
+    return x.reshape(-1, 3)
+
+
+def gen_maxwell(masses, kT, seed):
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    v = np.zeros(3 * len(m))
+    check(library().md_gen_maxwell(len(m), ptr(m), kT, seed, ptr(v)), "md_gen_maxwell")
+    return v.reshape(-1, 3)
+
+
+def gen_water(nmol, box, nl, roh, theta, jitter, seed):
+    b = np.asarray(box, dtype=np.float64)
+    x = np.zeros(9 * nmol)
+    check(library().md_gen_water(nmol, ptr(b), nl[0], nl[1], nl[2], roh, theta, jitter, seed, ptr(x)),
+          "md_gen_water")
+    return x.reshape(-1, 3)
+
+
+def gen_rsa(N, box, rmin, seed, max_trials=None):
+    b = np.asarray(box, dtype=np.float64)
+    x = np.zeros(3 * N)
+    n = check(library().md_gen_rsa(N, ptr(b), rmin, seed, max_trials or 200 * N, ptr(x)), "md_gen_rsa")
+    if n != N:
+        raise MdcoreError(f"RSA placed only {n} of {N} particles")
+    return x.reshape(-1, 3)
+
+
+def gen_types(N, ntypes, first, seed):
+    t = np.zeros(N, dtype=np.int32)
+    check(library().md_gen_types(N, ntypes, first, seed, ptr(t, C.c_int)), "md_gen_types")
+    return t
+
+
+def gen_uniform(n, lo, hi, seed):
+    out = np.zeros(n)
+    check(library().md_gen_uniform(n, lo, hi, seed, ptr(out)), "md_gen_uniform")
+    return out

@@ -1,0 +1,72 @@
+/*
+ * ORACLE — test infrastructure only.  A CPU, FP64 restatement of mdcore's
+ * nonbonded path used to check the GPU engine; nothing in the product links
+ * or calls it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.
+ */
+#ifndef MD_ORACLE_H
+#define MD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Window modes of the sorted pair loop. */
+#define OR_WINDOW_REF  0 /* reference: |shift|, unsigned arithmetic (runner_dopair.c:451-475) */
+#define OR_WINDOW_PROJ 1 /* projected shift.n, signed arithmetic */
+
+typedef struct or_system or_system;
+
+/* Tables: ntab tables; table t has n[t] intervals, alpha[4t..4t+3] and
+   coefficients c[off[t] .. off[t] + (n[t]+1)*8).  pind[ti*max_type+tj] is the
+   table of a type pair or -1. */
+typedef struct {
+    int max_type, ntab;
+    const int *pind;
+    const int *n;
+    const double *alpha;
+    const double *c;
+    const int *off;
+} or_tables;
+
+/* Build the cell grid and task list (space_init), insert the particles in
+   input order (engine_load) and apply one engine_shuffle.  x are absolute
+   positions, 3 per particle; particle ids are 0..N-1. */
+or_system *or_create(const double *origin, const double *dim, const double *L, double cutoff, int N,
+                     const double *x, const int *type, const or_tables *tab, int nexcl, const int *excl);
+void or_free(or_system *s);
+
+/* Sort every cell along its flagged directions (runner_dosort). */
+int or_sort(or_system *s);
+
+/* One nonbonded pass: sort, pair/self tasks, exclusions.  f[3*id+k] gets the
+   forces by particle id.  energies = {epot_nonbond, epot_exclusion, virial};
+   counts = {pairs evaluated, exclusions subtracted}.  nthreads > 1 runs the
+   tasks on pthreads with private force buffers (summation order then
+   depends on scheduling). */
+int or_nonbond(or_system *s, int mode, int nthreads, double *f, double *energies, long long *counts);
+
+/* All-pairs check with the reference's r^2 arithmetic but no window and no
+   excluded pairs:
+   counts every pair with r^2 <= rc^2 and a potential, and sums FP64 forces. */
+int or_bruteforce(or_system *s, double *f, double *energies, long long *count);
+
+/* Introspection. */
+int or_ncells(const or_system *s);
+int or_cell(const or_system *s, int cid, int *count, int *ids, unsigned int *sortflags);
+int or_sortlist(const or_system *s, int cid, int sid, unsigned int *out);
+int or_ntasks(const or_system *s, int *npair, int *nself);
+int or_task(const or_system *s, int k, int *type, int *i, int *j, int *sid);
+double or_dscale(const or_system *s);
+
+/* Building blocks, exported for unit tests. */
+void or_sort_desc(unsigned int *a, int N);
+unsigned int or_key(const double *x, const double *axis, double dscale, double bias);
+void or_eval(int n, const double *alpha, const double *c, double r2, double *e, double *f);
+extern const double or_axis[39];
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
