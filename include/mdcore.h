@@ -390,6 +390,8 @@ int engine_step(struct engine *e);
 int engine_nonbond_eval(struct engine *e);
 int engine_advance(struct engine *e);
 int engine_shuffle(struct engine *e);
+/* space_prepare(&e->s): zero forces and energies before a pass (new helper). */
+int engine_prepare(struct engine *e);
 int engine_exclusion_add(struct engine *e, int i, int j);
 int engine_exclusion_shrink(struct engine *e);
 int engine_finalize(struct engine *e);

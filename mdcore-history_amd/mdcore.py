@@ -75,7 +75,7 @@ def library():
         "engine_getenergies": (C.c_int, [C.c_void_p, _dp, _llp]),
         "engine_getsortlist": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up]),
         "engine_getcells": (C.c_int, [C.c_void_p, _ip, _ip]),
-        "space_prepare": (C.c_int, [C.c_void_p]),
+        "engine_prepare": (C.c_int, [C.c_void_p]),
         "engine_gpu_setdevice": (C.c_int, [C.c_void_p, C.c_int]),
         "engine_gpu_load": (C.c_int, [C.c_void_p]),
         "engine_nonbond_gpu": (C.c_int, [C.c_void_p]),
@@ -248,7 +248,7 @@ class Engine:
         check(self.lib.engine_shuffle(self.e), "engine_shuffle")
 
     def prepare(self):
-        check(self.lib.space_prepare(self.e), "space_prepare")
+        check(self.lib.engine_prepare(self.e), "engine_prepare")
 
     def nonbond(self):
         """One nonbonded pass (space_prepare + engine_nonbond_gpu), no integration."""

@@ -181,6 +181,12 @@ int engine_unload(struct engine *e, double *x, double *v, int *type, int *pid, i
     return count;
 }
 
+int engine_prepare(struct engine *e) {
+    if (e == NULL) return eerror_(engine_err_null);
+    if (space_prepare(&e->s) != space_err_ok) return eerror_(engine_err_space);
+    return engine_err_ok;
+}
+
 int engine_shuffle(struct engine *e) {
     if (space_shuffle_local(&e->s) < 0) return eerror_(engine_err_space);
     return engine_err_ok;

@@ -51,12 +51,15 @@ class System:
         return len(self.masses)
 
 
-def argon(n=20, a=0.565, seed=6178, T=100.0, tol=1e-3):
+def argon(n=20, a=0.565, seed=6178, T=100.0, tol=1e-3, jitter=0.0):
+    """jitter > 0 displaces every atom by U[-jitter/2, jitter/2] per axis."""
     eps, sigma, mass = 0.996, 0.3405, 39.948
     A = 4 * eps * sigma ** 12
     B = 4 * eps * sigma ** 6
     x = md.gen_fcc(n, n, n, a, a / 4)
     box = n * a
+    if jitter > 0:
+        x = np.mod(x + md.gen_uniform(x.size, -0.5 * jitter, 0.5 * jitter, seed + 1).reshape(x.shape), box)
     N = len(x)
     pot = md.LJ126(0.275, 1.0, A, B, tol)
     v = md.gen_maxwell(np.full(N, mass), KB * T, seed)
