@@ -13,17 +13,23 @@
  *               8 device queues (one per XCD slab of cells), walks the cell's
  *               neighbour list (the reference's self/pair tasks that touch
  *               it) and accumulates the forces of its own atoms only, so no
- *               atomics and no cross-wave write conflicts; pairs are tested
- *               with the reference's sorted-window predicate
- *               (runner_dopair.c:455-478) on the sorted lists staged in LDS,
- *               the cutoff test is re-done in FP64 inside a thin band
- *               around rc^2, the table is evaluated in FP32
- *               (potential_eval.h:67-104), and per-atom sums are two-sum
- *               compensated; excluded pairs are subtracted afterwards as in
+ *               atomics on forces and no cross-wave write conflicts.  Lanes
+ *               are (owner atom, partition); per neighbour cell the positions
+ *               and sorted words are prefetched one neighbour ahead and the
+ *               cell is staged in LDS in window order as packed records
+ *               (x, y, z, key | index << 16 | type << 26).  Each lane walks
+ *               its share of the sorted window (runner_dopair.c:455-478),
+ *               tests the cutoff (re-done in FP64 inside a thin band around
+ *               rc^2) and keeps its hits in a lane-private LDS list, then
+ *               evaluates them two at a time: FP32 table evaluation
+ *               (potential_eval.h:67-104), forces summed in registers and
+ *               folded into two-sum (hi, lo) accumulators per neighbour.
+ *               Excluded pairs are subtracted afterwards as in
  *               exclusion.c:81-282;
  *   k_reduce    per-cell FP64 energies/virials/counts -> totals, fixed order.
- * Every pair is evaluated by both of its cells (once per owner); energy,
- * virial and counts are taken on the task's i side only.
+ * Every pair is evaluated by both of its cells (once per owner) with
+ * bitwise-opposite dx, so the two owners see exactly opposite forces;
+ * energy, virial and counts are taken on the task's i side only.
  */
 #include <hip/hip_runtime.h>
 
@@ -34,48 +40,50 @@
 #include "engine_gpu.h"
 
 #define MD_NQUEUE 8
-#define MD_MAX_CELL 1024
-#define MD_MAX_TYPES 64
+#define MD_MAX_CELL 1024 /* particle index in a packed record: 10 bits */
+#define MD_MAX_TYPES 64  /* particle type in a packed record: 6 bits */
 
 #define gerror_(id, msg) errs_register((id), (msg), __LINE__, __func__, __FILE__)
 
-#define HIPCHECK(call)                                                             \
-    do {                                                                           \
-        hipError_t err_ = (call);                                                  \
-        if (err_ != hipSuccess) {                                                  \
+#define HIPCHECK(call)                                                                            \
+    do {                                                                                          \
+        hipError_t err_ = (call);                                                                 \
+        if (err_ != hipSuccess) {                                                                 \
             errs_register(engine_err_gpu, hipGetErrorString(err_), __LINE__, __func__, __FILE__); \
-            return engine_err_gpu;                                                 \
-        }                                                                          \
+            return engine_err_gpu;                                                                \
+        }                                                                                         \
     } while (0)
 
 /* ------------------------------------------------------------ parameters */
 
 struct nb_params {
-    int ncells, nparts, nids, max_type, ntab, proj;
+    int ncells, nparts, nids, max_type, proj;
     int dmaxdist;
     double dscale, bias, cutoff2;
     double dim[3], h[3];
     float rc2f, band;
+    int maxn, maxnbr;
     /* dynamic LDS layout (byte offsets) */
-    int lds_tab, lds_toff, lds_pos, lds_key;
+    int lds_own_key, lds_kfl, lds_nbe, lds_nbc, lds_stage, lds_hit;
 };
 
 /* Neighbour-list entry of an owner cell: x = other cell, y = sid | orient<<4
    | self<<5 (orient 0: owner is the task's i cell), z = dnshift with the
-   reference window |shift|, w = dnshift with the projected shift.n. */
+   reference window |shift|, w = dnshift with the projected shift.n.
+   Type-pair table header ptab[ti*mt+tj] = (alpha0, alpha1, alpha2,
+   bits(n | chunk0 << 9)) or w = bits(-1) when the pair has no potential. */
 
 struct gpu_state {
     int device;
     hipStream_t stream;
     hipEvent_t ev[8];
-    int ncells, ntab, max_type, nparts_cap, nids_cap, maxcount;
+    int ncells, ntab, max_type, nparts_cap, nids_cap, maxcount, maxnbr;
     int nparts, nids;
     struct nb_params P;
     int grid_nb, ncu;
     size_t lds_nb;
     /* static */
-    int *d_pind, *d_toff;
-    float4 *d_tab;
+    float4 *d_ptab;
     float *d_coeffs;
     int4 *d_nbr;
     int *d_nbr_off;
@@ -84,6 +92,7 @@ struct gpu_state {
     unsigned int *d_sortflags;
     int *d_excl_off, *d_excl_list;
     /* per pass */
+    int2 *d_cnt_off;
     int *d_count, *d_offset;
     double *d_x;
     int *d_type, *d_id, *d_index_of_id, *d_cell_of;
@@ -96,6 +105,7 @@ struct gpu_state {
     /* pinned host staging */
     double *h_x;
     int *h_type, *h_id, *h_count, *h_offset;
+    int2 *h_cnt_off;
     float4 *h_fhi, *h_flo;
     double *h_tot;
     unsigned long long *h_tot_cnt;
@@ -122,8 +132,7 @@ __device__ __forceinline__ double min_image(double s, double dim) {
 
 /* Reference r^2 of a pair task (runner_dopair.c:465-486 with fptype.h:275-278):
    pix = x_i - shift, dx = pix - x_j, r2 = dx0^2 + dx1^2 + dx2^2, no FMA. */
-__device__ __attribute__((noinline)) double r2_task64(const double *xi, const double *xj, double s0, double s1,
-                                                      double s2) {
+__device__ __forceinline__ double r2_task64(const double *xi, const double *xj, double s0, double s1, double s2) {
 #pragma clang fp contract(off)
     double p0 = xi[0] - s0, p1 = xi[1] - s1, p2 = xi[2] - s2;
     double d0 = p0 - xj[0], d1 = p1 - xj[1], d2 = p2 - xj[2];
@@ -131,8 +140,8 @@ __device__ __attribute__((noinline)) double r2_task64(const double *xi, const do
 }
 
 /* Reference r^2 of an exclusion (exclusion.c:139-147): i is the smaller id. */
-__device__ __attribute__((noinline)) double r2_excl64(const double *xi, const double *xj, const int *li,
-                                                      const int *lj, const double *h) {
+__device__ __forceinline__ double r2_excl64(const double *xi, const double *xj, const int *li, const int *lj,
+                                            const double *h) {
 #pragma clang fp contract(off)
     double r2 = 0.0;
     for (int k = 0; k < 3; k++) {
@@ -147,18 +156,20 @@ __device__ __attribute__((noinline)) double r2_excl64(const double *xi, const do
     return r2;
 }
 
-/* Table evaluation in FP32: e and f = (dE/dr)/r (potential_eval.h:67-104). */
-__device__ __forceinline__ void pot_eval32(const float4 tab, const float *__restrict__ c0, float r2, float &e,
+/* Table evaluation in FP32: e and f = (dE/dr)/r (potential_eval.h:67-104).
+   hdr = (alpha0, alpha1, alpha2, bits(n | chunk0 << 9)). */
+__device__ __forceinline__ void pot_eval32(const float4 hdr, const float4 *__restrict__ c4, float r2, float &e,
                                            float &f) {
-    float ir = __builtin_amdgcn_rsqf(r2);
-    float r = r2 * ir;
-    float fi = fmaxf(0.0f, tab.x + r * (tab.y + r * tab.z));
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ir;
+    const float fi = fmaxf(0.0f, hdr.x + r * (hdr.y + r * hdr.z));
+    const unsigned int w = __float_as_uint(hdr.w);
+    const int n = (int)(w & 511u);
     int ind = (int)fi;
-    int n = __float_as_int(tab.w);
     ind = ind > n ? n : ind;
-    const float4 *c = (const float4 *)(c0 + ind * 8);
-    float4 a = c[0], b = c[1];
-    float x = (r - a.x) * a.y;
+    const float4 *c = c4 + 2 * ((int)(w >> 9) + ind);
+    const float4 a = c[0], b = c[1];
+    const float x = (r - a.x) * a.y;
     float ee = a.z * x + a.w;
     float eff = a.z;
     eff = eff * x + ee;
@@ -173,11 +184,29 @@ __device__ __forceinline__ void pot_eval32(const float4 tab, const float *__rest
     f = eff * a.y * ir;
 }
 
+/* The pair kernel's workgroup is a single wavefront, and the LDS executes a
+   wavefront's instructions in order: data written by one lane is visible to
+   another lane's later read once this wave's LDS operations have completed.
+   This waits for LDS only (lgkmcnt(0)) and keeps the compiler from moving
+   memory accesses across it; unlike a workgroup barrier it does not drain
+   outstanding global loads, so prefetches stay in flight. */
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
+
+/* Packed staged record: key (16 bits) | index in cell (10 bits) | type (6 bits). */
+__device__ __forceinline__ unsigned int rec_key(unsigned int w) { return w & 0xffffu; }
+__device__ __forceinline__ int rec_idx(unsigned int w) { return (int)((w >> 16) & 1023u); }
+__device__ __forceinline__ int rec_type(unsigned int w) { return (int)(w >> 26); }
 
 /* ------------------------------------------------------------ k_pack */
 
@@ -268,7 +297,7 @@ __constant__ double c_axis[39];
 
 /* Projection key (runner_dosort.c:146-149): dscale*(bias + x.n) in FP64,
    left to right, no fused multiply-add, truncated to unsigned. */
-__device__ __attribute__((noinline)) unsigned int sort_key(const double *x, int sid, double dscale, double bias) {
+__device__ __forceinline__ unsigned int sort_key(const double *x, int sid, double dscale, double bias) {
 #pragma clang fp contract(off)
     double v = dscale * (bias + x[0] * c_axis[3 * sid] + x[1] * c_axis[3 * sid + 1] + x[2] * c_axis[3 * sid + 2]);
     return (unsigned int)v;
@@ -286,7 +315,8 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
     int n = count[cid];
     if (n == 0) return;
     int off = offset[cid];
-    unsigned int *list = sortlist + 13 * (size_t)off + (size_t)sid * n;
+    const size_t seg = 13 * (size_t)off + (size_t)sid * n;
+    unsigned int *list = sortlist + seg;
     for (int i = 0; i < n; i++) {
         unsigned int key = sort_key(&x[3 * (size_t)(off + i)], sid, P.dscale, P.bias);
         list[i] = ((unsigned int)i << 16) | key;
@@ -297,42 +327,40 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
 
 /* ------------------------------------------------------------ k_nonbond */
 
-struct lane_acc {
-    float hx, hy, hz, lx, ly, lz;
-    double epot, vir, eexcl;
-    unsigned int np, nref, nexcl;
+struct nb_out {
+    float4 *fhi, *flo;
+    double *cell_acc;
+    unsigned long long *cell_cnt;
 };
 
-__global__ __launch_bounds__(64) void k_nonbond(
-    struct nb_params P, const int *__restrict__ count, const int *__restrict__ offset,
-    const int4 *__restrict__ nbr, const int *__restrict__ nbr_off, const double *__restrict__ origin,
-    const int *__restrict__ loc, const float4 *__restrict__ xf, const double *__restrict__ x64,
-    const int *__restrict__ pid, const unsigned int *__restrict__ sortlist,
-    const unsigned short *__restrict__ keyof, const int *__restrict__ pind_g, const float4 *__restrict__ tab_g,
-    const int *__restrict__ toff_g, const float *__restrict__ coeffs, const int *__restrict__ excl_off,
+#define MD_HCAP 32 /* hits buffered per lane before they are evaluated */
+
+/* The occupancy hint keeps the kernel at 4 waves per SIMD (128 VGPRs); the
+   kernel is bound by instruction issue and latency, and 4 waves measured
+   faster than 3 without the hint. */
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbond(
+    struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
+    const int *__restrict__ nbr_off, const double *__restrict__ origin, const int *__restrict__ loc,
+    const float4 *__restrict__ xf, const double *__restrict__ x64, const int *__restrict__ pid,
+    const unsigned int *__restrict__ sortlist, const unsigned short *__restrict__ keyof,
+    const float4 *__restrict__ ptab, const float4 *__restrict__ coeffs4, const int *__restrict__ excl_off,
     const int *__restrict__ excl_list, const int *__restrict__ index_of_id, const int *__restrict__ cell_of,
-    float4 *__restrict__ fhi, float4 *__restrict__ flo, double *__restrict__ cell_acc,
-    unsigned long long *__restrict__ cell_cnt, int *__restrict__ queue) {
+    struct nb_out O, int *__restrict__ queue) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    int *s_pind = (int *)smem;
-    float4 *s_tab = (float4 *)(smem + P.lds_tab);
-    int *s_toff = (int *)(smem + P.lds_toff);
-    float4 *s_pos = (float4 *)(smem + P.lds_pos);
-    int2 *s_key = (int2 *)(smem + P.lds_key);
+    float4 *raw = (float4 *)smem;                                        /* [64]        */
+    unsigned short *own_key = (unsigned short *)(smem + P.lds_own_key);  /* [13*64]     */
+    int *kfl = (int *)(smem + P.lds_kfl);                                /* [26]        */
+    int4 *s_nbe = (int4 *)(smem + P.lds_nbe);                            /* [maxnbr]    */
+    int2 *s_nbc = (int2 *)(smem + P.lds_nbc);                            /* [maxnbr]    */
+    float4 *stage = (float4 *)(smem + P.lds_stage);                      /* [maxn]      */
+    unsigned short *hitl = (unsigned short *)(smem + P.lds_hit);         /* [HCAP*64]   */
     const int lane = threadIdx.x;
     const int mt = P.max_type;
 
-    for (int k = lane; k < mt * mt; k += 64) s_pind[k] = pind_g[k];
-    for (int k = lane; k < P.ntab; k += 64) {
-        s_tab[k] = tab_g[k];
-        s_toff[k] = toff_g[k];
-    }
-    __syncthreads();
-
     int q = blockIdx.x % MD_NQUEUE, tried = 0;
     while (tried < MD_NQUEUE) {
-        int lo_q = (int)(((long long)q * P.ncells) / MD_NQUEUE);
-        int hi_q = (int)(((long long)(q + 1) * P.ncells) / MD_NQUEUE);
+        const int lo_q = (int)(((long long)q * P.ncells) / MD_NQUEUE);
+        const int hi_q = (int)(((long long)(q + 1) * P.ncells) / MD_NQUEUE);
         int it = 0;
         if (lane == 0) it = atomicAdd(&queue[q], 1);
         it = __shfl(it, 0);
@@ -342,18 +370,28 @@ __global__ __launch_bounds__(64) void k_nonbond(
             continue;
         }
         const int c = lo_q + it;
-        const int nc = count[c];
+        const int2 co_c = cnt_off[c];
+        const int nc = co_c.x, offc = co_c.y;
         if (nc == 0) continue;
-        const int offc = offset[c];
         const double oc0 = origin[3 * c], oc1 = origin[3 * c + 1], oc2 = origin[3 * c + 2];
-        const int nchunk = (nc + 63) / 64;
-        double cell_e = 0.0, cell_v = 0.0, cell_x = 0.0;
-        unsigned long long cell_np = 0, cell_nref = 0, cell_nx = 0;
+        const int k0 = nbr_off[c], nn = nbr_off[c + 1] - k0;
 
+        /* Owner metadata: neighbour list, first/last key per direction. */
+        wave_lds_sync();
+        for (int k = lane; k < nn; k += 64) {
+            const int4 en = nbr[k0 + k];
+            s_nbe[k] = en;
+            s_nbc[k] = cnt_off[en.x];
+        }
+        if (lane < 26) {
+            const int sid = lane % 13;
+            kfl[lane] = (int)(sortlist[13 * (size_t)offc + (size_t)sid * nc + (lane < 13 ? 0 : nc - 1)] & 0xffffu);
+        }
+
+        const int nchunk = (nc + 63) / 64;
         for (int ch = 0; ch < nchunk; ch++) {
             const int i0 = (int)(((long long)ch * nc) / nchunk);
-            const int i1 = (int)(((long long)(ch + 1) * nc) / nchunk);
-            const int nsub = i1 - i0;
+            const int nsub = (int)(((long long)(ch + 1) * nc) / nchunk) - i0;
             const int NP = 64 / nsub;
             const int il = lane % nsub, part = lane / nsub;
             const bool active = part < NP;
@@ -361,128 +399,214 @@ __global__ __launch_bounds__(64) void k_nonbond(
             const int a = offc + myidx;
             const float4 xi = xf[a];
             const int ti = __float_as_int(xi.w);
-            lane_acc A;
-            A.hx = A.hy = A.hz = A.lx = A.ly = A.lz = 0.0f;
-            A.epot = A.vir = A.eexcl = 0.0;
-            A.np = A.nref = A.nexcl = 0;
 
-            for (int k = nbr_off[c]; k < nbr_off[c + 1]; k++) {
-                const int4 en = nbr[k];
-                const int cj = en.x, sid = en.y & 15, orient = (en.y >> 4) & 1, self = (en.y >> 5) & 1;
-                const int nj = count[cj];
+            wave_lds_sync();
+            if (lane < nsub)
+                for (int sid = 0; sid < 13; sid++)
+                    own_key[sid * 64 + lane] = keyof[(size_t)sid * P.nparts + offc + i0 + lane];
+            wave_lds_sync();
+
+            /* Prefetch the first neighbour: unsorted positions + sorted words. */
+            float4 pfx = make_float4(0.f, 0.f, 0.f, 0.f);
+            unsigned int pfw = 0;
+            {
+                const int4 en = s_nbe[0];
+                const int2 cj = s_nbc[0];
+                if (lane < cj.x) {
+                    pfx = xf[cj.y + lane];
+                    if (!(en.y & 32)) {
+                        const int sid = en.y & 15, orient = (en.y >> 4) & 1;
+                        pfw = sortlist[13 * (size_t)cj.y + (size_t)sid * cj.x + (orient ? lane : cj.x - 1 - lane)];
+                    }
+                }
+            }
+
+            float hx = 0.f, hy = 0.f, hz = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
+            double epot = 0.0, vir = 0.0, eexcl = 0.0;
+            unsigned int np = 0, nref = 0, nexcl = 0;
+
+            for (int k = 0; k < nn; k++) {
+                const int4 en = s_nbe[k];
+                const int2 cjc = s_nbc[k];
+                const int cj = en.x, nj = cjc.x, offj = cjc.y;
+                const int sid = en.y & 15, orient = (en.y >> 4) & 1, self = (en.y >> 5) & 1;
+
+                /* Stage the neighbour, in the order its window is walked:
+                   ascending keys when the owner is the task's i cell,
+                   descending otherwise, storage order for the self task.
+                   Record = (x, y, z, key | index << 16 | type << 26). */
+                wave_lds_sync();
+                if (lane < nj) raw[lane] = pfx;
+                wave_lds_sync();
+                if (lane < nj) {
+                    if (self)
+                        stage[lane] = make_float4(pfx.x, pfx.y, pfx.z,
+                                                  __uint_as_float(((unsigned int)lane << 16) |
+                                                                  ((unsigned int)__float_as_int(pfx.w) << 26)));
+                    else {
+                        const int idx = (int)(pfw >> 16) < nj ? (int)(pfw >> 16) : nj - 1;
+                        const float4 p = idx < 64 ? raw[idx] : xf[offj + idx];
+                        stage[lane] = make_float4(p.x, p.y, p.z,
+                                                  __uint_as_float((pfw & 0xffffu) | ((unsigned int)idx << 16) |
+                                                                  ((unsigned int)__float_as_int(p.w) << 26)));
+                    }
+                }
+                for (int t = lane + 64; t < nj; t += 64) {
+                    if (self) {
+                        const float4 p = xf[offj + t];
+                        stage[t] = make_float4(p.x, p.y, p.z,
+                                               __uint_as_float(((unsigned int)t << 16) |
+                                                               ((unsigned int)__float_as_int(p.w) << 26)));
+                    } else {
+                        const unsigned int w =
+                            sortlist[13 * (size_t)offj + (size_t)sid * nj + (orient ? t : nj - 1 - t)];
+                        const int idx = (int)(w >> 16) < nj ? (int)(w >> 16) : nj - 1;
+                        const float4 p = xf[offj + idx];
+                        stage[t] = make_float4(p.x, p.y, p.z,
+                                               __uint_as_float((w & 0xffffu) | ((unsigned int)idx << 16) |
+                                                               ((unsigned int)__float_as_int(p.w) << 26)));
+                    }
+                }
+                wave_lds_sync();
+
+                /* Prefetch the next neighbour while this one is processed. */
+                if (k + 1 < nn) {
+                    const int4 en1 = s_nbe[k + 1];
+                    const int2 c1 = s_nbc[k + 1];
+                    if (lane < c1.x) {
+                        pfx = xf[c1.y + lane];
+                        if (!(en1.y & 32)) {
+                            const int sid1 = en1.y & 15, or1 = (en1.y >> 4) & 1;
+                            pfw = sortlist[13 * (size_t)c1.y + (size_t)sid1 * c1.x + (or1 ? lane : c1.x - 1 - lane)];
+                        }
+                    }
+                }
                 if (nj == 0) continue;
-                const int offj = offset[cj];
-                __syncthreads();
-                if (self) {
-                    for (int t = lane; t < nj; t += 64) {
-                        s_pos[t] = xf[offj + t];
-                        s_key[t] = make_int2(0, t);
-                    }
-                } else {
-                    const unsigned int *sl = sortlist + 13 * (size_t)offj + (size_t)sid * nj;
-                    for (int t = lane; t < nj; t += 64) {
-                        unsigned int w = sl[orient ? t : nj - 1 - t];
-                        int idx = (int)(w >> 16);
-                        idx = idx < nj ? idx : nj - 1; /* sort wrote idx < nj; keep loads in bounds regardless */
-                        s_pos[t] = xf[offj + idx];
-                        s_key[t] = make_int2((int)(w & 0xffff), idx);
-                    }
-                }
-                __syncthreads();
 
-                /* Shift from the owner cell to the neighbour (minimum image). */
-                const double s0 = min_image(origin[3 * cj] - oc0, P.dim[0]);
-                const double s1 = min_image(origin[3 * cj + 1] - oc1, P.dim[1]);
-                const double s2 = min_image(origin[3 * cj + 2] - oc2, P.dim[2]);
-                const float sf0 = (float)s0, sf1 = (float)s1, sf2 = (float)s2;
-                float le = 0.0f, lv = 0.0f, lfx = 0.0f, lfy = 0.0f, lfz = 0.0f;
+                /* Shift from the owner cell to the neighbour (minimum image),
+                   rounded to FP32; the FP64 value is recomputed where needed. */
+                const float sf0 = (float)min_image(origin[3 * cj] - oc0, P.dim[0]);
+                const float sf1 = (float)min_image(origin[3 * cj + 1] - oc1, P.dim[1]);
+                const float sf2 = (float)min_image(origin[3 * cj + 2] - oc2, P.dim[2]);
+
+                int myk = 0, dns = 0;
+                bool live_ref = true, run = true;
+                if (!self) {
+                    dns = P.proj ? en.w : en.z;
+                    const int kstage0 = (int)(__float_as_uint(stage[0].w) & 0xffffu);
+                    const int s_imax = orient ? kstage0 : kfl[sid];
+                    const int s_jmin = orient ? kfl[13 + sid] : kstage0;
+                    live_ref = (s_jmin + en.z - s_imax) >= 0;
+                    run = P.proj || live_ref;
+                    myk = own_key[sid * 64 + il];
+                }
+
+                float lfx = 0.f, lfy = 0.f, lfz = 0.f, le = 0.f, lv = 0.f;
                 unsigned int lnp = 0, lnref = 0;
-
-                if (self) {
-                    if (active) {
-                        for (int t = part; t < nj; t += NP) {
-                            if (t == myidx) continue;
-                            const float4 pj = s_pos[t];
-                            const int tb = s_pind[ti * mt + __float_as_int(pj.w)];
-                            if (tb < 0) continue;
-                            const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
-                            const float r2 = dx * dx + dy * dy + dz * dz;
-                            bool in = r2 <= P.rc2f;
-                            if (fabsf(r2 - P.rc2f) <= P.band)
-                                in = r2_task64(&x64[3 * (size_t)a], &x64[3 * (size_t)(offj + t)], 0.0, 0.0, 0.0) <=
-                                     P.cutoff2;
-                            if (!in) continue;
-                            float e, f;
-                            pot_eval32(s_tab[tb], coeffs + s_toff[tb], r2, e, f);
-                            lfx -= f * dx;
-                            lfy -= f * dy;
-                            lfz -= f * dz;
-                            if (myidx > t) {
-                                le += e;
-                                lv -= f * r2;
-                                lnp++;
-                            }
+                int t = part;
+                bool more = active && run;
+                while (__any(more)) {
+                    /* Filter: walk this lane's part of the window, keep hits. */
+                    int h = 0;
+                    while (more && h < MD_HCAP) {
+                        if (t >= nj) {
+                            more = false;
+                            break;
                         }
+                        const float4 pj = stage[t];
+                        const unsigned int wj = __float_as_uint(pj.w);
+                        if (!self) {
+                            const int kj = (int)rec_key(wj);
+                            const int gap = orient ? (myk + dns - kj) : (kj + dns - myk);
+                            if (gap >= P.dmaxdist) {
+                                more = false;
+                                break;
+                            }
+                        } else if (t == myidx) {
+                            t += NP;
+                            continue;
+                        }
+                        const float dx = (xi.x - pj.x) - sf0, dy = (xi.y - pj.y) - sf1, dz = (xi.z - pj.z) - sf2;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        bool hit = r2 <= P.rc2f;
+                        if (fabsf(r2 - P.rc2f) <= P.band) {
+                            /* Decide in FP64 exactly as the reference does, in
+                               the task's own orientation. */
+                            const double *xa = &x64[3 * (size_t)a];
+                            const double *xb = &x64[3 * (size_t)(offj + rec_idx(wj))];
+                            const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cj];
+                            double r2d;
+                            if (self || orient == 0)
+                                r2d = r2_task64(xa, xb, min_image(o_nb[0] - o_own[0], P.dim[0]),
+                                                min_image(o_nb[1] - o_own[1], P.dim[1]),
+                                                min_image(o_nb[2] - o_own[2], P.dim[2]));
+                            else
+                                r2d = r2_task64(xb, xa, min_image(o_own[0] - o_nb[0], P.dim[0]),
+                                                min_image(o_own[1] - o_nb[1], P.dim[1]),
+                                                min_image(o_own[2] - o_nb[2], P.dim[2]));
+                            hit = r2d <= P.cutoff2;
+                        }
+                        if (hit) {
+                            hitl[h * 64 + lane] = (unsigned short)t;
+                            h++;
+                        }
+                        t += NP;
                     }
-                    lnref = lnp;
-                } else {
-                    const int dns = P.proj ? en.w : en.z;
-                    const unsigned int *slc = sortlist + 13 * (size_t)offc + (size_t)sid * nc;
-                    const int kfirst = (int)(slc[0] & 0xffff), klast = (int)(slc[nc - 1] & 0xffff);
-                    const int s_imax = orient ? s_key[0].x : kfirst;
-                    const int s_jmin = orient ? klast : s_key[0].x;
-                    const bool live_ref = (s_jmin + en.z - s_imax) >= 0;
-                    if ((P.proj || live_ref) && active) {
-                        const int myk = keyof[(size_t)sid * P.nparts + a];
-                        for (int t = part; t < nj; t += NP) {
-                            const int2 kj = s_key[t];
-                            const int gap = orient ? (myk + dns - kj.x) : (kj.x + dns - myk);
-                            if (gap >= P.dmaxdist) break;
-                            const float4 pj = s_pos[t];
-                            const int tb = s_pind[ti * mt + __float_as_int(pj.w)];
-                            if (tb < 0) continue;
-                            const float dx = (xi.x - pj.x) - sf0, dy = (xi.y - pj.y) - sf1,
-                                        dz = (xi.z - pj.z) - sf2;
-                            const float r2 = dx * dx + dy * dy + dz * dz;
-                            bool in = r2 <= P.rc2f;
-                            if (fabsf(r2 - P.rc2f) <= P.band) {
-                                const double *xa = &x64[3 * (size_t)a];
-                                const double *xb = &x64[3 * (size_t)(offj + kj.y)];
-                                double r2d;
-                                if (orient == 0)
-                                    r2d = r2_task64(xa, xb, s0, s1, s2);
-                                else
-                                    r2d = r2_task64(xb, xa, min_image(oc0 - origin[3 * cj], P.dim[0]),
-                                                    min_image(oc1 - origin[3 * cj + 1], P.dim[1]),
-                                                    min_image(oc2 - origin[3 * cj + 2], P.dim[2]));
-                                in = r2d <= P.cutoff2;
-                            }
-                            if (!in) continue;
-                            float e, f;
-                            pot_eval32(s_tab[tb], coeffs + s_toff[tb], r2, e, f);
-                            lfx -= f * dx;
-                            lfy -= f * dy;
-                            lfz -= f * dz;
-                            if (orient == 0) {
-                                le += e;
-                                lv -= f * r2;
-                                lnp++;
-                                lnref += (live_ref && (kj.x + en.z - myk) < P.dmaxdist) ? 1u : 0u;
-                            }
+
+                    /* Evaluate the kept hits, two per lane per round. */
+                    for (int s = 0; __any(s < h); s += 2) {
+                        const bool va = s < h, vb = s + 1 < h;
+                        const int ta = va ? hitl[s * 64 + lane] : 0;
+                        const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
+                        const float4 pa = stage[ta], pb = stage[tb];
+                        const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
+                        const float4 ha = ptab[ti * mt + rec_type(wa)], hb = ptab[ti * mt + rec_type(wb)];
+                        const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
+                        const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1, dzb = (xi.z - pb.z) - sf2;
+                        const float r2a = dxa * dxa + dya * dya + dza * dza;
+                        const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+                        const bool ua = va && __float_as_int(ha.w) != -1;
+                        const bool ub = vb && __float_as_int(hb.w) != -1;
+                        float ea = 0.f, fa = 0.f, eb = 0.f, fb = 0.f;
+                        if (ua) pot_eval32(ha, coeffs4, r2a, ea, fa);
+                        if (ub) pot_eval32(hb, coeffs4, r2b, eb, fb);
+                        if (ua) {
+                            lfx -= fa * dxa;
+                            lfy -= fa * dya;
+                            lfz -= fa * dza;
+                        }
+                        if (ub) {
+                            lfx -= fb * dxb;
+                            lfy -= fb * dyb;
+                            lfz -= fb * dzb;
+                        }
+                        const bool ca = ua && (self ? (myidx > rec_idx(wa)) : (orient == 0));
+                        const bool cb = ub && (self ? (myidx > rec_idx(wb)) : (orient == 0));
+                        if (ca) {
+                            le += ea;
+                            lv -= fa * r2a;
+                            lnp++;
+                            lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                        }
+                        if (cb) {
+                            le += eb;
+                            lv -= fb * r2b;
+                            lnp++;
+                            lnref += (self || (live_ref && ((int)rec_key(wb) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
                         }
                     }
                 }
-                acc2(A.hx, A.lx, lfx);
-                acc2(A.hy, A.ly, lfy);
-                acc2(A.hz, A.lz, lfz);
-                A.epot += (double)le;
-                A.vir += (double)lv;
-                A.np += lnp;
-                A.nref += lnref;
+                acc2(hx, lx, lfx);
+                acc2(hy, ly, lfy);
+                acc2(hz, lz, lfz);
+                epot += (double)le;
+                vir += (double)lv;
+                np += lnp;
+                nref += lnref;
             }
 
             /* Excluded pairs: subtract what the pair pass added
-               (exclusion.c:139-208), same FP32 arithmetic as above. */
+               (exclusion.c:139-208), same FP32 arithmetic as the pair pass. */
             if (active && part == 0) {
                 const int myid = pid[a];
                 for (int k = excl_off[myid]; k < excl_off[myid + 1]; k++) {
@@ -492,13 +616,12 @@ __global__ __launch_bounds__(64) void k_nonbond(
                     if (b < 0) continue;
                     const int cb = cell_of[b];
                     const float4 pj = xf[b];
-                    const int tb = s_pind[ti * mt + __float_as_int(pj.w)];
-                    if (tb < 0) continue;
-                    const double s0 = min_image(origin[3 * cb] - oc0, P.dim[0]);
-                    const double s1 = min_image(origin[3 * cb + 1] - oc1, P.dim[1]);
-                    const double s2 = min_image(origin[3 * cb + 2] - oc2, P.dim[2]);
-                    const float dx = (xi.x - pj.x) - (float)s0, dy = (xi.y - pj.y) - (float)s1,
-                                dz = (xi.z - pj.z) - (float)s2;
+                    const float4 hdr = ptab[ti * mt + __float_as_int(pj.w)];
+                    if (__float_as_int(hdr.w) == -1) continue;
+                    const float ex = (float)min_image(origin[3 * cb] - oc0, P.dim[0]);
+                    const float ey = (float)min_image(origin[3 * cb + 1] - oc1, P.dim[1]);
+                    const float ez = (float)min_image(origin[3 * cb + 2] - oc2, P.dim[2]);
+                    const float dx = (xi.x - pj.x) - ex, dy = (xi.y - pj.y) - ey, dz = (xi.z - pj.z) - ez;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     bool in = r2 <= P.rc2f;
                     if (fabsf(r2 - P.rc2f) <= P.band) {
@@ -509,14 +632,14 @@ __global__ __launch_bounds__(64) void k_nonbond(
                                        P.h) <= P.cutoff2;
                     }
                     if (!in) continue;
-                    float e, f;
-                    pot_eval32(s_tab[tb], coeffs + s_toff[tb], r2, e, f);
-                    acc2(A.hx, A.lx, f * dx);
-                    acc2(A.hy, A.ly, f * dy);
-                    acc2(A.hz, A.lz, f * dz);
+                    float e, ff;
+                    pot_eval32(hdr, coeffs4, r2, e, ff);
+                    acc2(hx, lx, ff * dx);
+                    acc2(hy, ly, ff * dy);
+                    acc2(hz, lz, ff * dz);
                     if (myid < oid) {
-                        A.eexcl -= (double)e;
-                        A.nexcl++;
+                        eexcl -= (double)e;
+                        nexcl++;
                     }
                 }
             }
@@ -524,39 +647,50 @@ __global__ __launch_bounds__(64) void k_nonbond(
             /* Fold the partitions of each atom into partition 0, in order. */
             for (int p = 1; p < NP; p++) {
                 const int src = (lane + p * nsub) & 63;
-                const float ohx = __shfl(A.hx, src), ohy = __shfl(A.hy, src), ohz = __shfl(A.hz, src);
-                const float olx = __shfl(A.lx, src), oly = __shfl(A.ly, src), olz = __shfl(A.lz, src);
+                const float ohx = __shfl(hx, src), ohy = __shfl(hy, src), ohz = __shfl(hz, src);
+                const float olx = __shfl(lx, src), oly = __shfl(ly, src), olz = __shfl(lz, src);
                 if (part == 0) {
-                    acc2(A.hx, A.lx, ohx);
-                    acc2(A.hy, A.ly, ohy);
-                    acc2(A.hz, A.lz, ohz);
-                    A.lx += olx;
-                    A.ly += oly;
-                    A.lz += olz;
+                    acc2(hx, lx, ohx);
+                    acc2(hy, ly, ohy);
+                    acc2(hz, lz, ohz);
+                    lx += olx;
+                    ly += oly;
+                    lz += olz;
                 }
             }
             if (part == 0) {
-                fhi[a] = make_float4(A.hx, A.hy, A.hz, 0.0f);
-                flo[a] = make_float4(A.lx, A.ly, A.lz, 0.0f);
+                O.fhi[a] = make_float4(hx, hy, hz, 0.0f);
+                O.flo[a] = make_float4(lx, ly, lz, 0.0f);
             }
             if (!active) {
-                A.epot = A.vir = A.eexcl = 0.0;
-                A.np = A.nref = A.nexcl = 0;
+                epot = vir = eexcl = 0.0;
+                np = nref = nexcl = 0;
             }
-            cell_e += wave_sum(A.epot);
-            cell_v += wave_sum(A.vir);
-            cell_x += wave_sum(A.eexcl);
-            cell_np += (unsigned long long)wave_sum((int)A.np);
-            cell_nref += (unsigned long long)wave_sum((int)A.nref);
-            cell_nx += (unsigned long long)wave_sum((int)A.nexcl);
-        }
-        if (lane == 0) {
-            cell_acc[3 * c] = cell_e;
-            cell_acc[3 * c + 1] = cell_x;
-            cell_acc[3 * c + 2] = cell_v;
-            cell_cnt[3 * c] = cell_np;
-            cell_cnt[3 * c + 1] = cell_nx;
-            cell_cnt[3 * c + 2] = cell_nref;
+
+            /* Per-cell sums; this wave owns the cell's slots, chunks add in order. */
+            epot = wave_sum(epot);
+            vir = wave_sum(vir);
+            eexcl = wave_sum(eexcl);
+            np = (unsigned int)wave_sum((int)np);
+            nref = (unsigned int)wave_sum((int)nref);
+            nexcl = (unsigned int)wave_sum((int)nexcl);
+            if (lane == 0) {
+                if (ch == 0) {
+                    O.cell_acc[3 * c] = epot;
+                    O.cell_acc[3 * c + 1] = eexcl;
+                    O.cell_acc[3 * c + 2] = vir;
+                    O.cell_cnt[3 * c] = np;
+                    O.cell_cnt[3 * c + 1] = nexcl;
+                    O.cell_cnt[3 * c + 2] = nref;
+                } else {
+                    O.cell_acc[3 * c] += epot;
+                    O.cell_acc[3 * c + 1] += eexcl;
+                    O.cell_acc[3 * c + 2] += vir;
+                    O.cell_cnt[3 * c] += np;
+                    O.cell_cnt[3 * c + 1] += nexcl;
+                    O.cell_cnt[3 * c + 2] += nref;
+                }
+            }
         }
     }
 }
@@ -683,9 +817,7 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipSetDevice(g->device);
     (void)hipStreamSynchronize(g->stream);
     gpu_free_pass(g);
-    (void)hipFree(g->d_pind);
-    (void)hipFree(g->d_toff);
-    (void)hipFree(g->d_tab);
+    (void)hipFree(g->d_ptab);
     (void)hipFree(g->d_coeffs);
     (void)hipFree(g->d_nbr);
     (void)hipFree(g->d_nbr_off);
@@ -695,6 +827,7 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_excl_off);
     (void)hipFree(g->d_excl_list);
     (void)hipFree(g->d_index_of_id);
+    (void)hipFree(g->d_cnt_off);
     (void)hipFree(g->d_cell_acc);
     (void)hipFree(g->d_cell_cnt);
     (void)hipFree(g->d_tot);
@@ -702,6 +835,7 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_queue);
     (void)hipHostFree(g->h_count);
     (void)hipHostFree(g->h_offset);
+    (void)hipHostFree(g->h_cnt_off);
     (void)hipHostFree(g->h_tot);
     (void)hipHostFree(g->h_tot_cnt);
     for (int k = 0; k < 8; k++) (void)hipEventDestroy(g->ev[k]);
@@ -733,7 +867,7 @@ static void task_dnshift(const struct space *s, int ti, int tj, int sid, double 
 extern "C" int engine_gpu_load(struct engine *e) {
     struct gpu_state *g;
     struct space *s;
-    int mt, k, i, j, ntab = 0, ncoef = 0, cid;
+    int mt, k, i, j, ntab = 0, nchunks = 0, cid;
     if (e == NULL) return gerror_(engine_err_null, "NULL engine");
     if (e->gpu != NULL) engine_gpu_finalize(e);
     s = &e->s;
@@ -748,48 +882,54 @@ extern "C" int engine_gpu_load(struct engine *e) {
     g->ncells = s->nr_cells;
     g->max_type = mt;
 
-    /* Tables: one device table per distinct potential, FP32-rounded. */
+    /* Tables: one device table per distinct potential, FP32-rounded, and a
+       header per type pair. */
     {
         struct potential **uniq = (struct potential **)calloc((size_t)mt * mt, sizeof(*uniq));
-        int *pind = (int *)malloc(sizeof(int) * mt * mt);
+        int *tabof = (int *)malloc(sizeof(int) * mt * mt);
+        int *chunk0 = (int *)malloc(sizeof(int) * mt * mt);
         for (k = 0; k < mt * mt; k++) {
             struct potential *p = e->p[k];
-            pind[k] = -1;
+            tabof[k] = -1;
             if (p == NULL) continue;
+            if (p->n < 1 || p->n > 511) return gerror_(engine_err_potential, "table with more than 511 intervals");
             for (j = 0; j < ntab; j++)
                 if (uniq[j] == p) break;
             if (j == ntab) {
+                chunk0[ntab] = nchunks;
                 uniq[ntab++] = p;
-                ncoef += (p->n + 1) * potential_chunk;
+                nchunks += p->n + 1;
             }
-            pind[k] = j;
+            tabof[k] = j;
         }
-        float4 *tab = (float4 *)malloc(sizeof(float4) * (ntab > 0 ? ntab : 1));
-        int *toff = (int *)malloc(sizeof(int) * (ntab > 0 ? ntab : 1));
-        float *coef = (float *)malloc(sizeof(float) * (ncoef > 0 ? ncoef : 8));
-        int pos = 0;
+        if (nchunks >= (1 << 22)) return gerror_(engine_err_potential, "tables too large");
+        float4 *ptab = (float4 *)malloc(sizeof(float4) * mt * mt);
+        float *coef = (float *)malloc(sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1));
         for (j = 0; j < ntab; j++) {
             struct potential *p = uniq[j];
-            int n = p->n;
-            tab[j] = make_float4((float)p->alpha[0], (float)p->alpha[1], (float)p->alpha[2], 0.0f);
-            memcpy(&tab[j].w, &n, sizeof(int));
-            toff[j] = pos;
-            for (k = 0; k < (n + 1) * potential_chunk; k++) coef[pos + k] = (float)p->c[k];
-            pos += (n + 1) * potential_chunk;
+            for (k = 0; k < (p->n + 1) * potential_chunk; k++) coef[8 * chunk0[j] + k] = (float)p->c[k];
+        }
+        for (k = 0; k < mt * mt; k++) {
+            if (tabof[k] < 0) {
+                int m1 = -1;
+                ptab[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                memcpy(&ptab[k].w, &m1, sizeof(int));
+            } else {
+                struct potential *p = uniq[tabof[k]];
+                unsigned int w = (unsigned int)p->n | ((unsigned int)chunk0[tabof[k]] << 9);
+                ptab[k] = make_float4((float)p->alpha[0], (float)p->alpha[1], (float)p->alpha[2], 0.f);
+                memcpy(&ptab[k].w, &w, sizeof(unsigned int));
+            }
         }
         g->ntab = ntab;
-        HIPCHECK(hipMalloc(&g->d_pind, sizeof(int) * mt * mt));
-        HIPCHECK(hipMalloc(&g->d_tab, sizeof(float4) * (ntab > 0 ? ntab : 1)));
-        HIPCHECK(hipMalloc(&g->d_toff, sizeof(int) * (ntab > 0 ? ntab : 1)));
-        HIPCHECK(hipMalloc(&g->d_coeffs, sizeof(float) * (ncoef > 0 ? ncoef : 8)));
-        HIPCHECK(hipMemcpy(g->d_pind, pind, sizeof(int) * mt * mt, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(g->d_tab, tab, sizeof(float4) * (ntab > 0 ? ntab : 1), hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(g->d_toff, toff, sizeof(int) * (ntab > 0 ? ntab : 1), hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(g->d_coeffs, coef, sizeof(float) * (ncoef > 0 ? ncoef : 8), hipMemcpyHostToDevice));
+        HIPCHECK(hipMalloc(&g->d_ptab, sizeof(float4) * mt * mt));
+        HIPCHECK(hipMalloc(&g->d_coeffs, sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1)));
+        HIPCHECK(hipMemcpy(g->d_ptab, ptab, sizeof(float4) * mt * mt, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(g->d_coeffs, coef, sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1), hipMemcpyHostToDevice));
         free(uniq);
-        free(pind);
-        free(tab);
-        free(toff);
+        free(tabof);
+        free(chunk0);
+        free(ptab);
         free(coef);
     }
 
@@ -799,7 +939,6 @@ extern "C" int engine_gpu_load(struct engine *e) {
         memset(P, 0, sizeof(*P));
         P->ncells = s->nr_cells;
         P->max_type = mt;
-        P->ntab = ntab;
         P->proj = (e->flags & engine_flag_projwindow) ? 1 : 0;
         P->bias = sqrt(s->h[0] * s->h[0] + s->h[1] * s->h[1] + s->h[2] * s->h[2]);
         P->dscale = (double)32767 / (2 * P->bias);
@@ -819,9 +958,9 @@ extern "C" int engine_gpu_load(struct engine *e) {
     /* Owner neighbour lists from the task list. */
     {
         int *cnt = (int *)calloc(s->nr_cells + 1, sizeof(int));
-        int *fill;
+        int *fill, *off;
         int4 *nbr;
-        int total = 0;
+        int total = 0, maxnbr = 1;
         for (k = 0; k < s->nr_tasks; k++) {
             struct task *t = &s->tasks[k];
             if (t->type == task_type_self)
@@ -831,12 +970,15 @@ extern "C" int engine_gpu_load(struct engine *e) {
                 cnt[t->j]++;
             }
         }
-        for (cid = 0; cid < s->nr_cells; cid++) total += cnt[cid];
+        for (cid = 0; cid < s->nr_cells; cid++) {
+            total += cnt[cid];
+            if (cnt[cid] > maxnbr) maxnbr = cnt[cid];
+        }
         nbr = (int4 *)malloc(sizeof(int4) * (total > 0 ? total : 1));
         fill = (int *)malloc(sizeof(int) * (s->nr_cells + 1));
+        off = (int *)malloc(sizeof(int) * (s->nr_cells + 1));
         fill[0] = 0;
         for (cid = 0; cid < s->nr_cells; cid++) fill[cid + 1] = fill[cid] + cnt[cid];
-        int *off = (int *)malloc(sizeof(int) * (s->nr_cells + 1));
         memcpy(off, fill, sizeof(int) * (s->nr_cells + 1));
         for (k = 0; k < s->nr_tasks; k++) {
             struct task *t = &s->tasks[k];
@@ -849,6 +991,8 @@ extern "C" int engine_gpu_load(struct engine *e) {
                 nbr[fill[t->j]++] = make_int4(t->i, t->flags | (1 << 4), dr, dp);
             }
         }
+        g->maxnbr = maxnbr;
+        g->P.maxnbr = maxnbr;
         HIPCHECK(hipMalloc(&g->d_nbr, sizeof(int4) * (total > 0 ? total : 1)));
         HIPCHECK(hipMalloc(&g->d_nbr_off, sizeof(int) * (s->nr_cells + 1)));
         HIPCHECK(hipMemcpy(g->d_nbr, nbr, sizeof(int4) * (total > 0 ? total : 1), hipMemcpyHostToDevice));
@@ -920,6 +1064,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
         if (gpu_alloc_ids(g, nids > 0 ? nids : 1) < 0) return engine_err_gpu;
     }
 
+    HIPCHECK(hipMalloc(&g->d_cnt_off, sizeof(int2) * s->nr_cells));
     HIPCHECK(hipMalloc(&g->d_cell_acc, sizeof(double) * 3 * s->nr_cells));
     HIPCHECK(hipMalloc(&g->d_cell_cnt, sizeof(unsigned long long) * 3 * s->nr_cells));
     HIPCHECK(hipMalloc(&g->d_tot, sizeof(double) * 4));
@@ -927,6 +1072,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
     HIPCHECK(hipMalloc(&g->d_queue, sizeof(int) * (MD_NQUEUE + 1)));
     HIPCHECK(hipHostMalloc((void **)&g->h_count, sizeof(int) * s->nr_cells, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_offset, sizeof(int) * (s->nr_cells + 1), hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void **)&g->h_cnt_off, sizeof(int2) * s->nr_cells, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_tot, sizeof(double) * 4, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_tot_cnt, sizeof(unsigned long long) * 4, hipHostMallocDefault));
 
@@ -943,22 +1089,24 @@ extern "C" int engine_gpu_load(struct engine *e) {
 static int gpu_upload(struct engine *e) {
     struct gpu_state *g = (struct gpu_state *)e->gpu;
     struct space *s = &e->s;
-    int cid, k, n = 0, maxc = 0;
+    int cid, k, n = 0, maxc = 0, maxid = -1;
     for (cid = 0; cid < s->nr_cells; cid++) {
         int c = s->cells[cid].count;
         if (c > maxc) maxc = c;
         n += c;
     }
     if (maxc > MD_MAX_CELL) return gerror_(engine_err_maxparts, "more than 1024 particles in a cell");
-    if (maxc >= 65536) return gerror_(engine_err_maxparts, "sortlist index exceeds 16 bits");
     if (gpu_alloc_parts(g, n > 0 ? n : 1) < 0) return engine_err_gpu;
-    int maxid = -1;
     for (cid = 0; cid < s->nr_cells; cid++) {
         struct cell *c = &s->cells[cid];
         g->h_count[cid] = c->count;
         g->h_offset[cid] = (cid == 0) ? 0 : g->h_offset[cid - 1] + s->cells[cid - 1].count;
+        g->h_cnt_off[cid].x = c->count;
+        g->h_cnt_off[cid].y = g->h_offset[cid];
         for (k = 0; k < c->count; k++) {
             int a = g->h_offset[cid] + k;
+            if (c->parts[k].type < 0 || c->parts[k].type >= g->max_type)
+                return gerror_(engine_err_range, "particle type out of range");
             g->h_x[3 * a] = c->parts[k].x[0];
             g->h_x[3 * a + 1] = c->parts[k].x[1];
             g->h_x[3 * a + 2] = c->parts[k].x[2];
@@ -973,32 +1121,39 @@ static int gpu_upload(struct engine *e) {
     g->maxcount = maxc;
     g->P.nparts = n;
     g->P.nids = g->nids;
+    g->P.maxn = maxc > 0 ? maxc : 1;
     HIPCHECK(hipMemcpyAsync(g->d_x, g->h_x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g->stream));
     HIPCHECK(hipMemcpyAsync(g->d_type, g->h_type, sizeof(int) * n, hipMemcpyHostToDevice, g->stream));
     HIPCHECK(hipMemcpyAsync(g->d_id, g->h_id, sizeof(int) * n, hipMemcpyHostToDevice, g->stream));
     HIPCHECK(hipMemcpyAsync(g->d_count, g->h_count, sizeof(int) * s->nr_cells, hipMemcpyHostToDevice, g->stream));
     HIPCHECK(hipMemcpyAsync(g->d_offset, g->h_offset, sizeof(int) * (s->nr_cells + 1), hipMemcpyHostToDevice,
                             g->stream));
+    HIPCHECK(hipMemcpyAsync(g->d_cnt_off, g->h_cnt_off, sizeof(int2) * s->nr_cells, hipMemcpyHostToDevice,
+                            g->stream));
     return engine_err_ok;
 }
 
-/* Launch the four kernels; optional events around each. */
+/* Launch the kernels of one pass; optional events around each. */
 static int gpu_launch(struct gpu_state *g, int timed) {
     struct nb_params P = g->P;
-    const int mt = g->max_type;
     int nb;
     size_t lds = 0;
-    lds += sizeof(int) * mt * mt;
+    const size_t maxn = (size_t)P.maxn;
+    lds = sizeof(float4) * 64; /* raw */
+    P.lds_own_key = (int)lds;
+    lds += sizeof(unsigned short) * 13 * 64;
     lds = (lds + 15) & ~(size_t)15;
-    P.lds_tab = (int)lds;
-    lds += sizeof(float4) * (g->ntab > 0 ? g->ntab : 1);
-    P.lds_toff = (int)lds;
-    lds += sizeof(int) * (g->ntab > 0 ? g->ntab : 1);
+    P.lds_kfl = (int)lds;
+    lds += sizeof(int) * 32;
+    P.lds_nbe = (int)lds;
+    lds += sizeof(int4) * (size_t)g->maxnbr;
+    P.lds_nbc = (int)lds;
+    lds += sizeof(int2) * (size_t)g->maxnbr;
     lds = (lds + 15) & ~(size_t)15;
-    P.lds_pos = (int)lds;
-    lds += sizeof(float4) * (g->maxcount > 0 ? g->maxcount : 1);
-    P.lds_key = (int)lds;
-    lds += sizeof(int2) * (g->maxcount > 0 ? g->maxcount : 1);
+    P.lds_stage = (int)lds;
+    lds += sizeof(float4) * maxn;
+    P.lds_hit = (int)lds;
+    lds += sizeof(unsigned short) * 64 * MD_HCAP;
     if (lds > 64 * 1024) return gerror_(engine_err_maxparts, "LDS staging exceeds 64 KiB");
     if (lds != g->lds_nb) {
         /* Persistent grid: as many single-wave blocks as can be resident. */
@@ -1019,11 +1174,15 @@ static int gpu_launch(struct gpu_state *g, int timed) {
     hipLaunchKernelGGL(k_sort, dim3(nb), dim3(256), 0, g->stream, P, g->d_count, g->d_offset, g->d_sortflags,
                        g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE);
     if (timed) HIPCHECK(hipEventRecord(g->ev[1], g->stream));
-    hipLaunchKernelGGL(k_nonbond, dim3(g->grid_nb), dim3(64), lds, g->stream, P, g->d_count, g->d_offset, g->d_nbr,
+    struct nb_out O;
+    O.fhi = g->d_fhi;
+    O.flo = g->d_flo;
+    O.cell_acc = g->d_cell_acc;
+    O.cell_cnt = g->d_cell_cnt;
+    hipLaunchKernelGGL(k_nonbond, dim3(g->grid_nb), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
                        g->d_nbr_off, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist, g->d_keyof,
-                       g->d_pind, g->d_tab, g->d_toff, g->d_coeffs, g->d_excl_off, g->d_excl_list,
-                       g->d_index_of_id, g->d_cell_of, g->d_fhi, g->d_flo, g->d_cell_acc, g->d_cell_cnt,
-                       g->d_queue);
+                       g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list, g->d_index_of_id,
+                       g->d_cell_of, O, g->d_queue);
     if (timed) HIPCHECK(hipEventRecord(g->ev[2], g->stream));
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, g->stream, g->ncells, g->d_cell_acc, g->d_cell_cnt, g->d_tot,
                        g->d_tot_cnt);
@@ -1078,7 +1237,6 @@ extern "C" int engine_nonbond_gpu(struct engine *e) {
     struct gpu_state *g;
     struct space *s;
     int cid, k, flag = 0;
-    double t0, t1;
     if (e == NULL) return gerror_(engine_err_null, "NULL engine");
     g = (struct gpu_state *)e->gpu;
     if (g == NULL) return gerror_(engine_err_nogpu, "engine_gpu_load was not called");
@@ -1100,9 +1258,6 @@ extern "C" int engine_nonbond_gpu(struct engine *e) {
         HIPCHECK(hipEventElapsedTime(&ms, g->ev[0], g->ev[3]));
         e->timers[engine_timer_gpu_kernels] = ms * 1e-3;
     }
-    t0 = 0.0;
-    (void)t0;
-    (void)t1;
     for (cid = 0; cid < s->nr_cells; cid++) {
         struct cell *c = &s->cells[cid];
         for (k = 0; k < c->count; k++) {

@@ -17,7 +17,8 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIBPATH = HERE / "lib" / "libmdcore_gpu.so"
+# MDCORE_LIB selects another build of the same library (e.g. a tuning variant).
+LIBPATH = Path(os.environ.get("MDCORE_LIB", HERE / "lib" / "libmdcore_gpu.so"))
 
 # engine flags (include/mdcore.h, reference src/engine.h:56-72)
 engine_flag_none = 0

@@ -100,3 +100,15 @@ def test_projected_window_finds_every_pair(case):
         assert err <= FORCE_RTOL
     finally:
         eng.close()
+
+
+def test_repeatable(case):
+    """Two passes over the same input give bitwise-identical forces/energy."""
+    name, sys, orc, eng = case
+    f1 = eng.forces().copy()
+    e1 = eng.gpu_results()
+    eng.nonbond()
+    f2 = eng.forces()
+    e2 = eng.gpu_results()
+    assert np.array_equal(f1, f2), f"{name}: forces differ between passes"
+    assert e1 == e2
