@@ -409,6 +409,8 @@ int engine_getpositions(struct engine *e, double *x, int N);
 int engine_getenergies(struct engine *e, double *energies, long long *counts);
 /* Copy the host sortlist of cell cid, direction sid, into out (count entries). */
 int engine_getsortlist(struct engine *e, int cid, int sid, unsigned int *out);
+/* The task list as rows (type, i, j, flags); returns the number of tasks. */
+int engine_gettasks(struct engine *e, int *out, int max);
 /* Per-cell geometry: counts[cid], and the particle ids in cell order (N entries). */
 int engine_getcells(struct engine *e, int *counts, int *ids);
 void potential_free(struct potential *p);

@@ -76,6 +76,7 @@ def library():
         "engine_getenergies": (C.c_int, [C.c_void_p, _dp, _llp]),
         "engine_getsortlist": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up]),
         "engine_getcells": (C.c_int, [C.c_void_p, _ip, _ip]),
+        "engine_gettasks": (C.c_int, [C.c_void_p, _ip, C.c_int]),
         "engine_prepare": (C.c_int, [C.c_void_p]),
         "engine_gpu_setdevice": (C.c_int, [C.c_void_p, C.c_int]),
         "engine_gpu_load": (C.c_int, [C.c_void_p]),
@@ -293,6 +294,12 @@ class Engine:
         ids = np.zeros(max(self.N, 1), dtype=np.int32)
         n = check(self.lib.engine_getcells(self.e, ptr(counts, C.c_int), ptr(ids, C.c_int)), "engine_getcells")
         return counts, ids[:n]
+
+    def tasks(self):
+        n = check(self.lib.engine_gettasks(self.e, None, 0), "engine_gettasks")
+        out = np.zeros((max(n, 1), 4), dtype=np.int32)
+        check(self.lib.engine_gettasks(self.e, ptr(out, C.c_int), n), "engine_gettasks")
+        return out[:n]
 
     def sortlist(self, cid, sid, count):
         out = np.zeros(max(count, 1), dtype=np.uint32)

@@ -367,6 +367,19 @@ int engine_getsortlist(struct engine *e, int cid, int sid, unsigned int *out) {
     return c->count;
 }
 
+int engine_gettasks(struct engine *e, int *out, int max) {
+    int k;
+    if (e == NULL) return eerror_(engine_err_null);
+    if (out != NULL)
+        for (k = 0; k < e->s.nr_tasks && k < max; k++) {
+            out[4 * k] = e->s.tasks[k].type;
+            out[4 * k + 1] = e->s.tasks[k].i;
+            out[4 * k + 2] = e->s.tasks[k].j;
+            out[4 * k + 3] = e->s.tasks[k].flags;
+        }
+    return e->s.nr_tasks;
+}
+
 int engine_getcells(struct engine *e, int *counts, int *ids) {
     int cid, k, n = 0;
     if (e == NULL) return eerror_(engine_err_null);

@@ -1,0 +1,103 @@
+"""Host library: the C-ABI exports, the reference's error behaviour, and the
+host data model (cell contents after load + shuffle) against the oracle."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mdcore as md
+import pyoracle as po
+import systems
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    """Names of the functions declared in include/*.h (prototypes ending in ');')."""
+    names = set()
+    decl = re.compile(r"^(?:extern\s+)?(?:const\s+)?(?:struct\s+\w+|unsigned\s+\w+|\w+)\s*\**\s*([A-Za-z_]\w*)\s*\(")
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        text = re.sub(r"^\s*#.*$", "", text, flags=re.M)
+        text = re.sub(r"__attribute__\s*\(\(.*?\)\)", "", text)
+        for stmt in re.findall(r"[^;{}]*?\)\s*;", text):
+            m = decl.match(" ".join(stmt.split()))
+            if m:
+                names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_function():
+    lib = md.library()
+    names = header_functions()
+    assert "engine_nonbond_gpu" in names and "engine_gpu_load" in names and "engine_init" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared in include/*.h but not exported: {missing}"
+
+
+def test_engine_init_rejects_bad_input():
+    lib = md.library()
+    e = lib.engine_alloc()
+    o = np.zeros(3)
+    d = np.full(3, 5.0)
+    L = np.full(3, 1.0)
+    assert lib.engine_init(e, None, md.ptr(d), md.ptr(L), 1.0, 7, 2, md.engine_flag_gpu) == -1  # engine_err_null
+    assert lib.engine_init(e, md.ptr(o), md.ptr(d), md.ptr(L), 1.0, 7, 0, md.engine_flag_gpu) == -6  # range
+    assert "engine_init" in md.errors()
+    lib.errs_clear()
+    lib.engine_free(e)
+
+
+def test_load_outside_box_and_start_without_gpu_flag():
+    e = md.Engine([0, 0, 0], [3.0] * 3, [1.0] * 3, 1.0, 1, md.engine_flag_none)
+    e.addtype(1.0)
+    with pytest.raises(md.MdcoreError) as ex:
+        e.load(np.array([[3.5, 0.1, 0.1]]), np.array([0]))
+    assert "space_addpart" in str(ex.value)
+    e.load(np.array([[0.5, 0.5, 0.5]]), np.array([0]))
+    with pytest.raises(md.MdcoreError) as ex:
+        e.start()
+    assert "(-22)" in str(ex.value)  # engine_err_nogpu: no CPU fallback
+    e.close()
+
+
+def test_addpot_and_type_ranges():
+    e = md.Engine([0, 0, 0], [3.0] * 3, [1.0] * 3, 1.0, 2, md.engine_flag_gpu)
+    assert e.addtype(1.0) == 1 and e.addtype(2.0) == 2   # type 0 is the reserved null type
+    with pytest.raises(md.MdcoreError):
+        e.addtype(3.0)                                     # max_type reached
+    p = md.LJ126(0.275, 1.0, 9.677e-6, 6.209e-3, 1e-3)
+    with pytest.raises(md.MdcoreError):
+        e.addpot(p, 1, 3)
+    e.addpot(p, 1, 2)
+    e.close()
+
+
+@pytest.mark.parametrize("make", [lambda: systems.argon(6, jitter=0.08), lambda: systems.mixture(2500, seed=6),
+                                  lambda: systems.water(1000, 3.1, (10, 10, 10), seed=2)])
+def test_host_cells_match_oracle(make):
+    """engine_load + engine_shuffle place particles in the same cells, in the
+    same order, as the oracle's restatement of space_addpart/shuffle."""
+    sysm = make()
+    eng = systems.build_engine(sysm, md.engine_flag_gpu, start=False)
+    eng.shuffle()
+    orc = po.from_system(sysm)
+    counts, ids = eng.cells(orc.ncells)
+    pos = 0
+    for cid in range(orc.ncells):
+        oids, _ = orc.cell(cid)
+        assert np.array_equal(ids[pos:pos + counts[cid]], oids)
+        pos += counts[cid]
+    assert pos == sysm.N
+    x = eng.positions()
+    assert np.abs(np.mod(x - sysm.x + 0.5 * sysm.dim, sysm.dim) - 0.5 * sysm.dim).max() < 1e-12
+    eng.close()
+
+
+def test_exclusions_are_symmetric_and_deduplicated():
+    e = md.Engine([0, 0, 0], [3.0] * 3, [1.0] * 3, 1.0, 1, md.engine_flag_gpu)
+    for i, j in [(3, 1), (1, 3), (2, 2), (5, 4), (4, 5), (1, 3)]:
+        e.exclusion_add(i, j)
+    e.exclusion_shrink()
+    e.close()
