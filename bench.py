@@ -91,11 +91,17 @@ def main():
     ap.add_argument("--window", default="ref", choices=["ref", "proj"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--engine-steps", type=int, default=3, help="full engine_step calls timed for ns/day")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times, as the "
+                         "known-size reference for the FETCH_SIZE / WRITE_SIZE counter scale")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU only; multi-GPU runs are not part of this build "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py: the nonbonded engine runs on one GPU; runs on more than one GPU are not supported "
+              "(domain decomposition across GPUs is a design note, not code).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     import mdcore as md
     import systems
@@ -165,6 +171,19 @@ def main():
         out["gpu_over_cpu"] = value / base["value"]
     eng.close()
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib):
+    """A plain streaming copy (torch elementwise kernel, float32): each launch
+    reads and writes exactly mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 5}), flush=True)
 
 
 def eng_cells(sysm):

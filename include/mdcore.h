@@ -401,6 +401,9 @@ int engine_finalize(struct engine *e);
 unsigned long engine_sizeof(void);
 struct engine *engine_alloc(void);
 void engine_free(struct engine *e);
+/* Set the time step used by engine_step's kick-drift (the reference's drivers
+   assign e->dt directly). */
+int engine_setdt(struct engine *e, double dt);
 /* Per-particle forces f[3*id+k] (host copy), by particle id. */
 int engine_getforces(struct engine *e, double *f, int N);
 /* Absolute positions x[3*id+k], by particle id. */

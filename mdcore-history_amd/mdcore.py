@@ -59,6 +59,7 @@ def library():
         "engine_alloc": (C.c_void_p, []),
         "engine_free": (None, [C.c_void_p]),
         "engine_sizeof": (C.c_ulong, []),
+        "engine_setdt": (C.c_int, [C.c_void_p, C.c_double]),
         "engine_init": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_uint, C.c_int, C.c_uint]),
         "engine_addtype": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_char_p, C.c_char_p]),
         "engine_addpot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
@@ -245,6 +246,9 @@ class Engine:
 
     def step(self):
         check(self.lib.engine_step(self.e), "engine_step")
+
+    def setdt(self, dt):
+        check(self.lib.engine_setdt(self.e, dt), "engine_setdt")
 
     def shuffle(self):
         check(self.lib.engine_shuffle(self.e), "engine_shuffle")

@@ -63,6 +63,13 @@ static double wall_seconds(void) {
 
 unsigned long engine_sizeof(void) { return sizeof(struct engine); }
 
+int engine_setdt(struct engine *e, double dt) {
+    if (e == NULL) return eerror_(engine_err_null);
+    if (!(dt >= 0.0)) return eerror_(engine_err_range);
+    e->dt = dt;
+    return engine_err_ok;
+}
+
 struct engine *engine_alloc(void) {
     struct engine *e;
     if (posix_memalign((void **)&e, 64, sizeof(struct engine)) != 0) return NULL;

@@ -1,0 +1,60 @@
+"""The benchmark systems at full size on the GPU against the FP64 oracle.
+
+Bit-exact: pair counts (reference window), exclusion counts, and every
+sorted list of every cell.  Floating point: force relative RMSE <= 1e-5 and
+energy relative error <= 1e-6 against FP64.  The oracle runs on up to 16
+host threads (a few seconds per system).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mdcore as md
+import pyoracle as po
+import systems
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+FULL = {
+    "mixture_1m": lambda: systems.mixture(1048576),
+    "water_100k": lambda: systems.water(),
+    "argon_32k": lambda: systems.argon(20, jitter=0.05),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FULL))
+def test_full_size_parity(name):
+    sysm = FULL[name]()
+    eng = systems.build_engine(sysm, md.engine_flag_gpu)
+    eng.shuffle()
+    eng.nonbond()
+    g = eng.gpu_results()
+    f = eng.forces()
+    eng.sync(md.engine_gpu_sync_sort_down)
+    orc = po.from_system(sysm)
+    o = orc.nonbond(po.OR_WINDOW_REF, THREADS)
+    print(f"{name}: pairs {g['pairs']} (oracle {o['pairs']}), exclusions {g['exclusions']}", flush=True)
+    assert g["pairs"] == o["pairs"]
+    assert g["exclusions"] == o["exclusions"]
+    err = float(np.sqrt(np.sum((f - o["f"]) ** 2) / np.sum(o["f"] ** 2)))
+    print(f"{name}: force rel RMSE {err:.3e}, epot {g['epot']:.9e} vs {o['epot']:.9e}", flush=True)
+    assert err <= 1e-5
+    assert abs(g["epot"] - o["epot"]) <= 1e-6 * abs(o["epot"])
+    assert abs(g["epot_exclusion"] - o["epot_exclusion"]) <= 1e-6 * max(1.0, abs(o["epot_exclusion"]))
+    # sorted lists: every flagged direction of every cell (sampled cells for 1M)
+    orc.sort()
+    counts, ids = eng.cells(orc.ncells)
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    rng = np.random.default_rng(0)
+    cells = range(orc.ncells) if orc.ncells <= 2000 else rng.choice(orc.ncells, 2000, replace=False)
+    for cid in cells:
+        oids, flags = orc.cell(cid)
+        n = counts[cid]
+        assert np.array_equal(ids[starts[cid]:starts[cid] + n], oids)
+        for sid in range(13):
+            if n and (flags >> sid) & 1:
+                assert np.array_equal(eng.sortlist(cid, sid, n), orc.sortlist(cid, sid, n)), (cid, sid)
+    eng.close()
+    orc.close()

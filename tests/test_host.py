@@ -101,3 +101,12 @@ def test_exclusions_are_symmetric_and_deduplicated():
         e.exclusion_add(i, j)
     e.exclusion_shrink()
     e.close()
+
+
+def test_setdt_rejects_negative():
+    e = md.Engine([0, 0, 0], [3.0] * 3, [1.0] * 3, 1.0, 1, md.engine_flag_gpu)
+    e.setdt(0.002)
+    with pytest.raises(md.MdcoreError):
+        e.setdt(-1.0)
+    md.library().errs_clear()
+    e.close()
