@@ -13,7 +13,14 @@ passes timed with HIP events on the engine's stream.  Prints one JSON line.
                 evaluated pair over its mean launch time, against the FP32
                 peak; traffic from the committed rocprofv3 counter summary
   cpu_baseline  the FP64 oracle (CPU restatement of the reference runner)
-                on the host cores, one full pass of the same system
+                on the host cores, full passes of the same system; its forces
+                are the FP64 reference of force_rmse
+  mlp           the surrogate pass (engine_flag_mlp) on the same system:
+                trained on the GPU, timed like the table pass, its force
+                error against FP64, and a table set rebuilt at the tolerance
+                whose force error matches the network's, timed the same way
+  nve           energy drift of 1,000 NVE steps (dt 2 fs) of the 32,000-atom
+                argon system with the table pass and with a trained network
 """
 import argparse
 import json
@@ -29,6 +36,8 @@ sys.path.insert(0, str(ROOT / "mdcore-history_amd"))
 
 FLOP_PER_PAIR = 46          # algorithmic FLOP per evaluated pair interaction
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 peak
+MLP_MFMA_FLOP = 2 * 64 * 64  # 64x64 layer on MFMA per network evaluation
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
 DT_PS = 0.002               # 2 fs time step for ns/day
 METRIC = "pair-interactions/s & ns/day at 1M atoms; MLP vs table at fixed force RMSE"
 TRAFFIC_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_traffic.json"
@@ -76,7 +85,7 @@ def cpu_baseline(sysm, pairs_gpu, mode):
             break
     dt = time.perf_counter() - t0
     orc.close()
-    return {"value": pairs / dt, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+    return r["f"], {"value": pairs / dt, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
             "sample": f"{reps} full FP64 nonbonded passes (sort + pair/self + exclusions) of the same system, "
                       f"{r['pairs']} pairs each, {threads} threads on {model}; pair count equals the GPU's: "
                       f"{r['pairs'] == pairs_gpu}"}
@@ -90,6 +99,8 @@ def main():
     ap.add_argument("--workload", default="mixture_1m", choices=["mixture_1m", "water_100k", "argon_32k"])
     ap.add_argument("--window", default="ref", choices=["ref", "proj"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate section")
+    ap.add_argument("--no-nve", action="store_true", help="skip the NVE drift section")
     ap.add_argument("--engine-steps", type=int, default=3, help="full engine_step calls timed for ns/day")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times, as the "
@@ -134,16 +145,6 @@ def main():
                 "kernel": "k_nonbond", "kernel_ms": round(ms["nonbond_ms"], 4),
                 "note": "FP32 vector pipe (VALU); packed FP32 disabled, so 78.65 TFLOP/s is the issue ceiling"}
 
-    # Full engine_step (host pack, PCIe both ways, device pass, host kick-drift).
-    ns_day_full = None
-    if args.engine_steps > 0:
-        md.check(eng.lib.engine_step(eng.e), "engine_step")  # warm-up
-        t0 = time.perf_counter()
-        for _ in range(args.engine_steps):
-            md.check(eng.lib.engine_step(eng.e), "engine_step")
-        full_s = (time.perf_counter() - t0) / args.engine_steps
-        ns_day_full = DT_PS * 1e-3 * 86400.0 / full_s
-
     out = {
         "metric": METRIC,
         "value": value,
@@ -160,17 +161,162 @@ def main():
         "config": {"workload": f"{args.workload}: {desc}", "atoms": int(sysm.N), "pairs_per_step": int(pairs),
                    "window": args.window, "cells": int(eng_cells(sysm)), "parallelism": "single GPU"},
         "ns_per_day": DT_PS * 1e-3 * 86400.0 / step_s,
-        "ns_per_day_engine_step": ns_day_full,
+        "ns_per_day_engine_step": None,
         "kernels_ms": {k: round(v, 4) for k, v in ms.items()},
         "roofline": roofline,
         "setup_s": round(t_setup, 1),
     }
+    f_table = None
+    if not args.no_cpu_baseline or not args.no_mlp:
+        eng.nonbond()
+        f_table = eng.forces()
+        eng.sync(md.engine_gpu_sync_parts_up)
+    f64 = None
     if not args.no_cpu_baseline:
-        base = cpu_baseline(sysm, pairs, 0 if args.window == "ref" else 1)
+        f64, base = cpu_baseline(sysm, pairs, 0 if args.window == "ref" else 1)
         out["cpu_baseline"] = base
         out["gpu_over_cpu"] = value / base["value"]
+        out["force_rmse_vs_fp64"] = {"table": rel_rmse(f_table, f64)}
+    if not args.no_mlp:
+        out["mlp"] = mlp_section(sysm, eng, args, f_table, f64, value)
+        if f64 is not None:
+            out["force_rmse_vs_fp64"]["mlp"] = out["mlp"].pop("_rmse_fp64")
+        else:
+            out["mlp"].pop("_rmse_fp64")
+    # Full engine_step (host pack, PCIe both ways, device pass, host
+    # kick-drift); last, because it moves the atoms.
+    if args.engine_steps > 0:
+        eng.setflags(0, md.engine_flag_mlp)
+        md.check(eng.lib.engine_step(eng.e), "engine_step")  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(args.engine_steps):
+            md.check(eng.lib.engine_step(eng.e), "engine_step")
+        full_s = (time.perf_counter() - t0) / args.engine_steps
+        out["ns_per_day_engine_step"] = DT_PS * 1e-3 * 86400.0 / full_s
+    if not args.no_nve:
+        out["nve"] = nve_section()
     eng.close()
     print(json.dumps(out), flush=True)
+
+
+def rel_rmse(a, b):
+    import numpy as np
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def mlp_section(sysm, eng, args, f_table, f64, table_value):
+    """Train the surrogate on this system (half uniform-in-r samples, half the
+    configuration's own pair distances; FP64 table targets), then time the
+    surrogate pass exactly like the table pass."""
+    import numpy as np
+    import mdcore as md
+    import surrogate as sg
+    import systems
+    t0 = time.perf_counter()
+    dom = sg.pair_domains(eng, sysm)
+    S = sg.make_samples(eng, sysm, 1 << 22, dom[0], dom[1], configs=[sysm.x], seed=5)
+    t1 = time.perf_counter()
+    eng.mlp_init(*dom, seed=7)
+    hist = eng.mlp_train(*S, steps=20000, batch_tiles=1024, lr0=1e-2, lr1=1e-5, seed=11)
+    t2 = time.perf_counter()
+    eng.setflags(md.engine_flag_mlp, 0)
+    eng.nonbond()
+    f_mlp = eng.forces()
+    eng.sync(md.engine_gpu_sync_parts_up)
+    ms = eng.gpu_time(args.steps, args.warmup)
+    res = eng.gpu_results()
+    eng.setflags(0, md.engine_flag_mlp)
+    # share of the pairs below r_lo (evaluated from the tables), from a sample
+    # of the configuration's pair distances
+    _, ti, tj = sg.pair_potentials(eng, sysm)
+    pid = -np.ones((sysm.max_type, sysm.max_type), np.int64)
+    for k, (a, b) in enumerate(zip(ti, tj)):
+        pid[a, b] = pid[b, a] = k
+    r, a, b = md.sample_pairs(sysm.x, sysm.types, sysm.dim, float(sysm.cutoff), 1 << 20, 3)
+    pp = pid[a, b]
+    frac_table = float(np.mean(r[pp >= 0] < dom[0][pp[pp >= 0]]))
+    evals = 2.0 * res["pairs"] * (1.0 - frac_table)     # both owner cells evaluate a pair
+    achieved = MLP_MFMA_FLOP * evals / (ms["nonbond_ms"] * 1e-3) / 1e12
+    out = {
+        "value": res["pairs"] / (ms["total_ms"] * 1e-3), "unit": "pair-interactions/s",
+        "ms_per_step": ms["total_ms"], "kernels_ms": {k: round(v, 4) for k, v in ms.items()},
+        "ns_per_day": DT_PS * 1e-3 * 86400.0 / (ms["total_ms"] * 1e-3),
+        "vs_table": res["pairs"] / (ms["total_ms"] * 1e-3) / table_value,
+        "force_rmse_vs_table": rel_rmse(f_mlp, f_table),
+        "pairs": res["pairs"], "pairs_below_r_lo_share": round(frac_table, 5),
+        "training": {"samples": int(len(S[0])), "steps": 20000, "batch": 1024 * 16,
+                     "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
+                     "sample_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": None, "kernel": "k_nonbond_mlp",
+                     "kernel_ms": round(ms["nonbond_ms"], 4),
+                     "note": "bf16 MFMA FLOP of the 64x64 layer per network evaluation; VALU work (SiLU, "
+                             "layer 1, heads, pair search) not counted"},
+        "_rmse_fp64": rel_rmse(f_mlp, f64) if f64 is not None else None,
+    }
+    if sysm.name == "mixture":
+        out["table_at_matched_rmse"] = matched_table(sysm, f_table, out["force_rmse_vs_table"], args)
+    return out
+
+
+def matched_table(sysm, f_table, target, args):
+    """Rebuild the tables at coarser tolerances; keep the one whose force
+    error against the default tables is closest to target; time its pass."""
+    import math
+    import mdcore as md
+    import systems
+    best = None
+    for tol in (1e-2, 3e-2, 1e-1, 3e-1):
+        s2 = systems.with_tolerance(sysm, tol)
+        e2 = systems.build_engine(s2, md.engine_flag_gpu)
+        e2.shuffle()
+        e2.nonbond()
+        err = rel_rmse(e2.forces(), f_table)
+        if best is None or abs(math.log(err / target)) < abs(math.log(best[1] / target)):
+            if best is not None:
+                best[2].close()
+            best = (tol, err, e2)
+        else:
+            e2.close()
+    tol, err, e2 = best
+    e2.sync(md.engine_gpu_sync_parts_up)
+    ms = e2.gpu_time(args.steps, args.warmup)
+    pairs = e2.gpu_results()["pairs"]
+    e2.close()
+    return {"tol": tol, "force_rmse_vs_table": err, "value": pairs / (ms["total_ms"] * 1e-3),
+            "unit": "pair-interactions/s", "ms_per_step": ms["total_ms"]}
+
+
+def nve_section(steps=1000, dt=0.002):
+    """Energy drift of the 32,000-atom argon system over `steps` NVE steps,
+    table pass vs surrogate pass (network trained on the table's trajectory)."""
+    import mdcore as md
+    import surrogate as sg
+    import systems
+    sysm = systems.argon(20)
+    masses = sysm.masses[sysm.types]
+    t0 = time.perf_counter()
+    eng = systems.build_engine(sysm, md.engine_flag_gpu)
+    e_tab = sg.nve_energies(eng, masses, steps, dt)
+    x_end = eng.positions()
+    eng.close()
+    t1 = time.perf_counter()
+    eng = systems.build_engine(sysm, md.engine_flag_gpu)
+    dom = sg.pair_domains(eng, sysm)
+    S = sg.make_samples(eng, sysm, 1 << 20, dom[0], dom[1], configs=[x_end], seed=5)
+    eng.mlp_init(*dom, seed=7)
+    eng.mlp_train(*S, steps=10000, batch_tiles=512, lr0=1e-2, lr1=1e-5, seed=11)
+    eng.setflags(md.engine_flag_mlp, 0)
+    e_mlp = sg.nve_energies(eng, masses, steps, dt)
+    eng.close()
+    t2 = time.perf_counter()
+    rt, at = sg.drift_per_ns(e_tab, dt)
+    rm, am = sg.drift_per_ns(e_mlp, dt)
+    return {"system": "argon_32k: 32,000 LJ atoms, fcc start at 100 K, rc 1.0 nm", "steps": steps, "dt_ps": dt,
+            "table": {"drift_rel_per_ns": rt, "drift_kJ_mol_per_ns": at, "e_mean": float(e_tab.mean()),
+                      "e_std": float(e_tab.std()), "wall_s": round(t1 - t0, 1)},
+            "mlp": {"drift_rel_per_ns": rm, "drift_kJ_mol_per_ns": am, "e_mean": float(e_mlp.mean()),
+                    "e_std": float(e_mlp.std()), "wall_s": round(t2 - t1, 1)}}
 
 
 def copy_calibration(mib):

@@ -72,6 +72,53 @@ int engine_gpu_time(struct engine *e, int nsteps, int nwarm, double *ms);
    engine_flag_projwindow is set). */
 int engine_gpu_results(struct engine *e, double *out, long long *counts);
 
+/* ------------------------------------------------------------------------
+   MLP surrogate of the pair potential.  No reference counterpart: the
+   reference evaluates only its piecewise-polynomial tables
+   (potential_eval.h:67-104), which remain the judge and the fallback.
+
+   A species pair is a distinct potential of the engine; pair ids number
+   them in the order engine_gpu_load found them.  Per pair p the network
+   covers r in [r_lo[p], r_hi[p]] and predicts E = escale[p] sinh(y0) and
+   dE/dr = fscale[p] sinh(y1) from inputs (r, 1/r) and an 8-dim learned pair
+   embedding: 64 -> 64 hidden units (SiLU), bf16 weights on MFMA with fp32
+   accumulation for the 64x64 layer, and a 64 -> 2 head per pair.
+   Parameter vector: fp32, engine_mlp_init returns its length.  With
+   engine_flag_mlp set, engine_nonbond_gpu evaluates the pairs with
+   r_lo[p] <= r <= cutoff through the network and the others through the
+   tables. */
+#define engine_flag_mlp 131072
+
+/* Species pairs: ti[p], tj[p] (ti <= tj) for p < returned count. */
+int engine_mlp_pairs(struct engine *e, int *ti, int *tj, int max);
+
+/* Create the network for every species pair with the given domains and
+   output scales and seeded random weights; returns the parameter count. */
+int engine_mlp_init(struct engine *e, const double *r_lo, const double *r_hi, const double *escale,
+                    const double *fscale, unsigned long long seed);
+
+/* Copy the fp32 parameters out (get != 0) or in (get == 0); theta == NULL
+   returns the parameter count. */
+int engine_mlp_params(struct engine *e, float *theta, int get);
+
+/* Train on n samples (r, pair id, target E, target dE/dr) on the GPU: steps
+   Adam steps over shuffled tiles of 16 same-pair samples, batch_tiles tiles
+   per step, learning rate cosine-decayed from lr0 to lr1; loss per sample
+   wE (y0 - asinh(E/escale))^2 + wF (y1 - asinh(F/fscale))^2.  loss_hist[steps] (may be
+   NULL) receives the mean loss of each step's batch. */
+int engine_mlp_train(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
+                     long long n, int steps, int batch_tiles, double lr0, double lr1, double wE, double wF,
+                     unsigned long long seed, float *loss_hist);
+
+/* Gradient of the mean loss over n samples (n a multiple of 16, each run of
+   16 samples one pair id) with respect to the parameters, without an
+   update; grad has engine_mlp_params(e, NULL, 0) entries. */
+int engine_mlp_grad(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
+                    long long n, double wE, double wF, float *grad, double *loss);
+
+/* Evaluate the network on n samples: E[k] and dE/dr F[k]. */
+int engine_mlp_eval(struct engine *e, const float *r, const int *pair, long long n, float *E, float *F);
+
 #ifdef __cplusplus
 }
 #endif

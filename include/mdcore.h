@@ -136,6 +136,8 @@ struct potential *potential_create_LJ126_RF(double a, double b, double A, double
 
 /* Host FP64 evaluation of a table (potential_eval.h:67-104): e and f=(dE/dr)/r. */
 void potential_eval(const struct potential *p, FPTYPE r2, FPTYPE *e, FPTYPE *f);
+/* New: e[k] and dE/dr[k] at distances r[k], k < n. */
+int potential_eval_r(const struct potential *p, const double *r, long long n, double *e, double *dedr);
 
 /* Analytic helpers (potential.c:66-267). */
 double potential_LJ126(double r, double A, double B);
@@ -404,10 +406,15 @@ void engine_free(struct engine *e);
 /* Set the time step used by engine_step's kick-drift (the reference's drivers
    assign e->dt directly). */
 int engine_setdt(struct engine *e, double dt);
+/* Set and clear bits of e->flags (the reference's drivers assign the field);
+   engine_flag_gpu cannot be changed after engine_start. */
+int engine_setflags(struct engine *e, unsigned int set, unsigned int clear);
 /* Per-particle forces f[3*id+k] (host copy), by particle id. */
 int engine_getforces(struct engine *e, double *f, int N);
 /* Absolute positions x[3*id+k], by particle id. */
 int engine_getpositions(struct engine *e, double *x, int N);
+/* Velocities v[3*id+k], by particle id. */
+int engine_getvelocities(struct engine *e, double *v, int N);
 /* energies[0..3] = epot, epot_nonbond, epot_exclusion, virial; counts[0..1] = pairs, exclusions. */
 int engine_getenergies(struct engine *e, double *energies, long long *counts);
 /* Copy the host sortlist of cell cid, direction sid, into out (count entries). */

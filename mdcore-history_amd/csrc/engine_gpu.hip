@@ -37,79 +37,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "engine_gpu.h"
-
-#define MD_NQUEUE 8
-#define MD_MAX_CELL 1024 /* particle index in a packed record: 10 bits */
-#define MD_MAX_TYPES 64  /* particle type in a packed record: 6 bits */
-
-#define gerror_(id, msg) errs_register((id), (msg), __LINE__, __func__, __FILE__)
-
-#define HIPCHECK(call)                                                                            \
-    do {                                                                                          \
-        hipError_t err_ = (call);                                                                 \
-        if (err_ != hipSuccess) {                                                                 \
-            errs_register(engine_err_gpu, hipGetErrorString(err_), __LINE__, __func__, __FILE__); \
-            return engine_err_gpu;                                                                \
-        }                                                                                         \
-    } while (0)
-
-/* ------------------------------------------------------------ parameters */
-
-struct nb_params {
-    int ncells, nparts, nids, max_type, proj;
-    int dmaxdist;
-    double dscale, bias, cutoff2;
-    double dim[3], h[3];
-    float rc2f, band;
-    int maxn, maxnbr;
-    /* dynamic LDS layout (byte offsets) */
-    int lds_own_key, lds_kfl, lds_nbe, lds_nbc, lds_stage, lds_hit;
-};
-
-/* Neighbour-list entry of an owner cell: x = other cell, y = sid | orient<<4
-   | self<<5 (orient 0: owner is the task's i cell), z = dnshift with the
-   reference window |shift|, w = dnshift with the projected shift.n.
-   Type-pair table header ptab[ti*mt+tj] = (alpha0, alpha1, alpha2,
-   bits(n | chunk0 << 9)) or w = bits(-1) when the pair has no potential. */
-
-struct gpu_state {
-    int device;
-    hipStream_t stream;
-    hipEvent_t ev[8];
-    int ncells, ntab, max_type, nparts_cap, nids_cap, maxcount, maxnbr;
-    int nparts, nids;
-    struct nb_params P;
-    int grid_nb, ncu;
-    size_t lds_nb;
-    /* static */
-    float4 *d_ptab;
-    float *d_coeffs;
-    int4 *d_nbr;
-    int *d_nbr_off;
-    double *d_origin;
-    int *d_loc;
-    unsigned int *d_sortflags;
-    int *d_excl_off, *d_excl_list;
-    /* per pass */
-    int2 *d_cnt_off;
-    int *d_count, *d_offset;
-    double *d_x;
-    int *d_type, *d_id, *d_index_of_id, *d_cell_of;
-    float4 *d_xf, *d_fhi, *d_flo;
-    unsigned int *d_sortlist;
-    unsigned short *d_keyof;
-    double *d_cell_acc, *d_tot;
-    unsigned long long *d_cell_cnt, *d_tot_cnt;
-    int *d_queue;
-    /* pinned host staging */
-    double *h_x;
-    int *h_type, *h_id, *h_count, *h_offset;
-    int2 *h_cnt_off;
-    float4 *h_fhi, *h_flo;
-    double *h_tot;
-    unsigned long long *h_tot_cnt;
-};
+#include "gpu_state.h"
 
 /* ------------------------------------------------------------ device helpers */
 
@@ -333,20 +261,61 @@ struct nb_out {
     unsigned long long *cell_cnt;
 };
 
-#define MD_HCAP 32 /* hits buffered per lane before they are evaluated */
+/* Exclusive prefix sum of v over the wave. */
+__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x - v;
+}
 
-/* The occupancy hint keeps the kernel at 4 waves per SIMD (128 VGPRs); the
-   kernel is bound by instruction issue and latency, and 4 waves measured
-   faster than 3 without the hint. */
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbond(
-    struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
+/* Evaluate the dense pair list [0, T) with the network, 16 pairs per tile;
+   f = (dE/dr)/r and e of pair q go to dn_f[q], dn_e[q].  All 64 lanes must
+   call this together (T is wave-uniform). */
+__device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const float *dn_r2, const int *dn_p,
+                                               float *dn_f, float *dn_e, int T, int lane) {
+    for (int base = 0; base < T; base += 16) {
+        const int q = base + (lane & 15);
+        const int qv = q < T ? q : T - 1;
+        const float r = sqrtf(dn_r2[qv]);
+        const int p = dn_p[qv];
+        float z1[16], z2[16], h2[16], y0, y1;
+        mlp_bf16x8 h1b[2];
+        mlp_forward16(N, r, p, lane, z1, h1b, z2, h2, y0, y1);
+        if (lane < 16 && q < T) {
+            float e, f;
+            mlp_energy_force(N.sc[p], r, y0, y1, e, f);
+            dn_f[q] = f;
+            dn_e[q] = e;
+        }
+    }
+}
+
+#define MD_HCAP 32     /* table mode: hits buffered per lane before they are evaluated */
+#define MD_HCAP_MLP 8  /* MLP mode: hits per lane per compaction round */
+#define MD_XCAP_MLP 8  /* MLP mode: excluded partners per lane per round */
+
+/* The pair kernel body, for both evaluation modes:
+   MLP = false  every hit is evaluated from the FP32 tables by its lane;
+   MLP = true   hits below the species pair's r_lo (or without a network)
+                are evaluated from the tables, the others are compacted
+                wave-wide into a dense LDS pair list and evaluated by the
+                surrogate network 16 pairs per MFMA tile; each lane then
+                adds its own contiguous run of results in order, so the
+                result stays deterministic.  Excluded pairs take the same
+                route, so a pair and its exclusion cancel exactly. */
+template <bool MLP>
+__device__ __forceinline__ void nonbond_body(
+    unsigned char *smem, struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
     const int *__restrict__ nbr_off, const double *__restrict__ origin, const int *__restrict__ loc,
     const float4 *__restrict__ xf, const double *__restrict__ x64, const int *__restrict__ pid,
     const unsigned int *__restrict__ sortlist, const unsigned short *__restrict__ keyof,
     const float4 *__restrict__ ptab, const float4 *__restrict__ coeffs4, const int *__restrict__ excl_off,
     const int *__restrict__ excl_list, const int *__restrict__ index_of_id, const int *__restrict__ cell_of,
-    struct nb_out O, int *__restrict__ queue) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    struct nb_out O, int *__restrict__ queue, const struct mlp_net &N, const int *__restrict__ ptype) {
+    constexpr int HCAP = MLP ? MD_HCAP_MLP : MD_HCAP;
     float4 *raw = (float4 *)smem;                                        /* [64]        */
     unsigned short *own_key = (unsigned short *)(smem + P.lds_own_key);  /* [13*64]     */
     int *kfl = (int *)(smem + P.lds_kfl);                                /* [26]        */
@@ -354,6 +323,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     int2 *s_nbc = (int2 *)(smem + P.lds_nbc);                            /* [maxnbr]    */
     float4 *stage = (float4 *)(smem + P.lds_stage);                      /* [maxn]      */
     unsigned short *hitl = (unsigned short *)(smem + P.lds_hit);         /* [HCAP*64]   */
+    /* MLP mode: dense pair list (r^2, species pair) and its results (f, e),
+       and the excluded partners of each lane. */
+    float *dn_r2 = (float *)(smem + P.lds_dense);                        /* [HCAP*64]   */
+    int *dn_p = (int *)(smem + P.lds_dense + 4 * 64 * MD_HCAP_MLP);
+    float *dn_f = (float *)(smem + P.lds_dense + 8 * 64 * MD_HCAP_MLP);
+    float *dn_e = (float *)(smem + P.lds_dense + 12 * 64 * MD_HCAP_MLP);
+    int *exb = (int *)(smem + P.lds_dense + 16 * 64 * MD_HCAP_MLP);      /* [XCAP*64]   */
     const int lane = threadIdx.x;
     const int mt = P.max_type;
 
@@ -508,7 +484,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                 while (__any(more)) {
                     /* Filter: walk this lane's part of the window, keep hits. */
                     int h = 0;
-                    while (more && h < MD_HCAP) {
+                    while (more && h < HCAP) {
                         if (t >= nj) {
                             more = false;
                             break;
@@ -553,47 +529,108 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                         t += NP;
                     }
 
-                    /* Evaluate the kept hits, two per lane per round. */
-                    for (int s = 0; __any(s < h); s += 2) {
-                        const bool va = s < h, vb = s + 1 < h;
-                        const int ta = va ? hitl[s * 64 + lane] : 0;
-                        const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
-                        const float4 pa = stage[ta], pb = stage[tb];
-                        const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
-                        const float4 ha = ptab[ti * mt + rec_type(wa)], hb = ptab[ti * mt + rec_type(wb)];
-                        const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
-                        const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1, dzb = (xi.z - pb.z) - sf2;
-                        const float r2a = dxa * dxa + dya * dya + dza * dza;
-                        const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                        const bool ua = va && __float_as_int(ha.w) != -1;
-                        const bool ub = vb && __float_as_int(hb.w) != -1;
-                        float ea = 0.f, fa = 0.f, eb = 0.f, fb = 0.f;
-                        if (ua) pot_eval32(ha, coeffs4, r2a, ea, fa);
-                        if (ub) pot_eval32(hb, coeffs4, r2b, eb, fb);
-                        if (ua) {
+                    if constexpr (!MLP) {
+                        /* Evaluate the kept hits, two per lane per round. */
+                        for (int s = 0; __any(s < h); s += 2) {
+                            const bool va = s < h, vb = s + 1 < h;
+                            const int ta = va ? hitl[s * 64 + lane] : 0;
+                            const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
+                            const float4 pa = stage[ta], pb = stage[tb];
+                            const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
+                            const float4 ha = ptab[ti * mt + rec_type(wa)], hb = ptab[ti * mt + rec_type(wb)];
+                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
+                            const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1, dzb = (xi.z - pb.z) - sf2;
+                            const float r2a = dxa * dxa + dya * dya + dza * dza;
+                            const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+                            const bool ua = va && __float_as_int(ha.w) != -1;
+                            const bool ub = vb && __float_as_int(hb.w) != -1;
+                            float ea = 0.f, fa = 0.f, eb = 0.f, fb = 0.f;
+                            if (ua) pot_eval32(ha, coeffs4, r2a, ea, fa);
+                            if (ub) pot_eval32(hb, coeffs4, r2b, eb, fb);
+                            if (ua) {
+                                lfx -= fa * dxa;
+                                lfy -= fa * dya;
+                                lfz -= fa * dza;
+                            }
+                            if (ub) {
+                                lfx -= fb * dxb;
+                                lfy -= fb * dyb;
+                                lfz -= fb * dzb;
+                            }
+                            const bool ca = ua && (self ? (myidx > rec_idx(wa)) : (orient == 0));
+                            const bool cb = ub && (self ? (myidx > rec_idx(wb)) : (orient == 0));
+                            if (ca) {
+                                le += ea;
+                                lv -= fa * r2a;
+                                lnp++;
+                                lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                            }
+                            if (cb) {
+                                le += eb;
+                                lv -= fb * r2b;
+                                lnp++;
+                                lnref += (self || (live_ref && ((int)rec_key(wb) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                            }
+                        }
+                    } else {
+                        /* Below the pair's r_lo (or no network): table, by this lane.
+                           Otherwise keep the hit for the dense list. */
+                        int nd = 0;
+                        for (int s = 0; s < h; s++) {
+                            const int ta = hitl[s * 64 + lane];
+                            const float4 pa = stage[ta];
+                            const unsigned int wa = __float_as_uint(pa.w);
+                            const float4 ha = ptab[ti * mt + rec_type(wa)];
+                            if (__float_as_int(ha.w) == -1) continue;
+                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
+                            const float r2a = dxa * dxa + dya * dya + dza * dza;
+                            const float rlo = N.sc[ptype[ti * mt + rec_type(wa)]].x;
+                            if (r2a >= rlo * rlo) {
+                                hitl[nd * 64 + lane] = (unsigned short)ta;
+                                nd++;
+                                continue;
+                            }
+                            float ea, fa;
+                            pot_eval32(ha, coeffs4, r2a, ea, fa);
                             lfx -= fa * dxa;
                             lfy -= fa * dya;
                             lfz -= fa * dza;
+                            if (self ? (myidx > rec_idx(wa)) : (orient == 0)) {
+                                le += ea;
+                                lv -= fa * r2a;
+                                lnp++;
+                                lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                            }
                         }
-                        if (ub) {
-                            lfx -= fb * dxb;
-                            lfy -= fb * dyb;
-                            lfz -= fb * dzb;
+                        /* Dense list: this lane's hits at [off, off + nd). */
+                        const int off = wave_excl_scan(nd, lane);
+                        const int T = __shfl(off + nd, 63);
+                        for (int s = 0; s < nd; s++) {
+                            const float4 pa = stage[hitl[s * 64 + lane]];
+                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
+                            dn_r2[off + s] = dxa * dxa + dya * dya + dza * dza;
+                            dn_p[off + s] = ptype[ti * mt + rec_type(__float_as_uint(pa.w))];
                         }
-                        const bool ca = ua && (self ? (myidx > rec_idx(wa)) : (orient == 0));
-                        const bool cb = ub && (self ? (myidx > rec_idx(wb)) : (orient == 0));
-                        if (ca) {
-                            le += ea;
-                            lv -= fa * r2a;
-                            lnp++;
-                            lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                        wave_lds_sync();
+                        mlp_dense_eval(N, dn_r2, dn_p, dn_f, dn_e, T, lane);
+                        wave_lds_sync();
+                        for (int s = 0; s < nd; s++) {
+                            const float4 pa = stage[hitl[s * 64 + lane]];
+                            const unsigned int wa = __float_as_uint(pa.w);
+                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
+                            const float r2a = dxa * dxa + dya * dya + dza * dza;
+                            const float fa = dn_f[off + s], ea = dn_e[off + s];
+                            lfx -= fa * dxa;
+                            lfy -= fa * dya;
+                            lfz -= fa * dza;
+                            if (self ? (myidx > rec_idx(wa)) : (orient == 0)) {
+                                le += ea;
+                                lv -= fa * r2a;
+                                lnp++;
+                                lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                            }
                         }
-                        if (cb) {
-                            le += eb;
-                            lv -= fb * r2b;
-                            lnp++;
-                            lnref += (self || (live_ref && ((int)rec_key(wb) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
-                        }
+                        wave_lds_sync();
                     }
                 }
                 acc2(hx, lx, lfx);
@@ -607,40 +644,120 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 
             /* Excluded pairs: subtract what the pair pass added
                (exclusion.c:139-208), same FP32 arithmetic as the pair pass. */
-            if (active && part == 0) {
-                const int myid = pid[a];
-                for (int k = excl_off[myid]; k < excl_off[myid + 1]; k++) {
-                    const int oid = excl_list[k];
-                    if (oid == myid) continue;
-                    const int b = index_of_id[oid];
-                    if (b < 0) continue;
+            if constexpr (!MLP) {
+                if (active && part == 0) {
+                    const int myid = pid[a];
+                    for (int k = excl_off[myid]; k < excl_off[myid + 1]; k++) {
+                        const int oid = excl_list[k];
+                        if (oid == myid) continue;
+                        const int b = index_of_id[oid];
+                        if (b < 0) continue;
+                        const int cb = cell_of[b];
+                        const float4 pj = xf[b];
+                        const float4 hdr = ptab[ti * mt + __float_as_int(pj.w)];
+                        if (__float_as_int(hdr.w) == -1) continue;
+                        const float ex = (float)min_image(origin[3 * cb] - oc0, P.dim[0]);
+                        const float ey = (float)min_image(origin[3 * cb + 1] - oc1, P.dim[1]);
+                        const float ez = (float)min_image(origin[3 * cb + 2] - oc2, P.dim[2]);
+                        const float dx = (xi.x - pj.x) - ex, dy = (xi.y - pj.y) - ey, dz = (xi.z - pj.z) - ez;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        bool in = r2 <= P.rc2f;
+                        if (fabsf(r2 - P.rc2f) <= P.band) {
+                            const bool mine = myid < oid;
+                            const int ia = mine ? a : b, ib = mine ? b : a;
+                            const int ca = mine ? c : cb, cbb = mine ? cb : c;
+                            in = r2_excl64(&x64[3 * (size_t)ia], &x64[3 * (size_t)ib], &loc[3 * ca], &loc[3 * cbb],
+                                           P.h) <= P.cutoff2;
+                        }
+                        if (!in) continue;
+                        float e, ff;
+                        pot_eval32(hdr, coeffs4, r2, e, ff);
+                        acc2(hx, lx, ff * dx);
+                        acc2(hy, ly, ff * dy);
+                        acc2(hz, lz, ff * dz);
+                        if (myid < oid) {
+                            eexcl -= (double)e;
+                            nexcl++;
+                        }
+                    }
+                }
+            } else {
+                const bool xl = active && part == 0;
+                const int myid = xl ? pid[a] : 0;
+                int kx = xl ? excl_off[myid] : 0;
+                const int kend = xl ? excl_off[myid + 1] : 0;
+                /* dx and r^2 of this lane's atom to slot b, as in the pair pass. */
+                auto excl_geom = [&](int b, float &dx, float &dy, float &dz) {
                     const int cb = cell_of[b];
                     const float4 pj = xf[b];
-                    const float4 hdr = ptab[ti * mt + __float_as_int(pj.w)];
-                    if (__float_as_int(hdr.w) == -1) continue;
-                    const float ex = (float)min_image(origin[3 * cb] - oc0, P.dim[0]);
-                    const float ey = (float)min_image(origin[3 * cb + 1] - oc1, P.dim[1]);
-                    const float ez = (float)min_image(origin[3 * cb + 2] - oc2, P.dim[2]);
-                    const float dx = (xi.x - pj.x) - ex, dy = (xi.y - pj.y) - ey, dz = (xi.z - pj.z) - ez;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    bool in = r2 <= P.rc2f;
-                    if (fabsf(r2 - P.rc2f) <= P.band) {
-                        const bool mine = myid < oid;
-                        const int ia = mine ? a : b, ib = mine ? b : a;
-                        const int ca = mine ? c : cb, cbb = mine ? cb : c;
-                        in = r2_excl64(&x64[3 * (size_t)ia], &x64[3 * (size_t)ib], &loc[3 * ca], &loc[3 * cbb],
-                                       P.h) <= P.cutoff2;
+                    dx = (xi.x - pj.x) - (float)min_image(origin[3 * cb] - oc0, P.dim[0]);
+                    dy = (xi.y - pj.y) - (float)min_image(origin[3 * cb + 1] - oc1, P.dim[1]);
+                    dz = (xi.z - pj.z) - (float)min_image(origin[3 * cb + 2] - oc2, P.dim[2]);
+                    return __float_as_int(pj.w);
+                };
+                while (__any(kx < kend)) {
+                    int nd = 0;
+                    while (kx < kend && nd < MD_XCAP_MLP) {
+                        const int oid = excl_list[kx++];
+                        if (oid == myid) continue;
+                        const int b = index_of_id[oid];
+                        if (b < 0) continue;
+                        float dx, dy, dz;
+                        const int tb = excl_geom(b, dx, dy, dz);
+                        const float4 hdr = ptab[ti * mt + tb];
+                        if (__float_as_int(hdr.w) == -1) continue;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        bool in = r2 <= P.rc2f;
+                        if (fabsf(r2 - P.rc2f) <= P.band) {
+                            const int cb = cell_of[b];
+                            const bool mine = myid < oid;
+                            const int ia = mine ? a : b, ib = mine ? b : a;
+                            const int ca = mine ? c : cb, cbb = mine ? cb : c;
+                            in = r2_excl64(&x64[3 * (size_t)ia], &x64[3 * (size_t)ib], &loc[3 * ca], &loc[3 * cbb],
+                                           P.h) <= P.cutoff2;
+                        }
+                        if (!in) continue;
+                        const float rlo = N.sc[ptype[ti * mt + tb]].x;
+                        if (r2 >= rlo * rlo) {
+                            exb[nd * 64 + lane] = b;
+                            nd++;
+                            continue;
+                        }
+                        float e, ff;
+                        pot_eval32(hdr, coeffs4, r2, e, ff);
+                        acc2(hx, lx, ff * dx);
+                        acc2(hy, ly, ff * dy);
+                        acc2(hz, lz, ff * dz);
+                        if (myid < oid) {
+                            eexcl -= (double)e;
+                            nexcl++;
+                        }
                     }
-                    if (!in) continue;
-                    float e, ff;
-                    pot_eval32(hdr, coeffs4, r2, e, ff);
-                    acc2(hx, lx, ff * dx);
-                    acc2(hy, ly, ff * dy);
-                    acc2(hz, lz, ff * dz);
-                    if (myid < oid) {
-                        eexcl -= (double)e;
-                        nexcl++;
+                    const int off = wave_excl_scan(nd, lane);
+                    const int T = __shfl(off + nd, 63);
+                    for (int s = 0; s < nd; s++) {
+                        float dx, dy, dz;
+                        const int tb = excl_geom(exb[s * 64 + lane], dx, dy, dz);
+                        dn_r2[off + s] = dx * dx + dy * dy + dz * dz;
+                        dn_p[off + s] = ptype[ti * mt + tb];
                     }
+                    wave_lds_sync();
+                    mlp_dense_eval(N, dn_r2, dn_p, dn_f, dn_e, T, lane);
+                    wave_lds_sync();
+                    for (int s = 0; s < nd; s++) {
+                        const int b = exb[s * 64 + lane];
+                        float dx, dy, dz;
+                        excl_geom(b, dx, dy, dz);
+                        const float ff = dn_f[off + s], e = dn_e[off + s];
+                        acc2(hx, lx, ff * dx);
+                        acc2(hy, ly, ff * dy);
+                        acc2(hz, lz, ff * dz);
+                        if (myid < pid[b]) {
+                            eexcl -= (double)e;
+                            nexcl++;
+                        }
+                    }
+                    wave_lds_sync();
                 }
             }
 
@@ -693,6 +810,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             }
         }
     }
+}
+
+#define NB_PARAMS                                                                                             \
+    struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,                       \
+        const int *__restrict__ nbr_off, const double *__restrict__ origin, const int *__restrict__ loc,      \
+        const float4 *__restrict__ xf, const double *__restrict__ x64, const int *__restrict__ pid,          \
+        const unsigned int *__restrict__ sortlist, const unsigned short *__restrict__ keyof,                \
+        const float4 *__restrict__ ptab, const float4 *__restrict__ coeffs4, const int *__restrict__ excl_off, \
+        const int *__restrict__ excl_list, const int *__restrict__ index_of_id, const int *__restrict__ cell_of, \
+        struct nb_out O, int *__restrict__ queue
+#define NB_ARGS                                                                                              \
+    P, cnt_off, nbr, nbr_off, origin, loc, xf, x64, pid, sortlist, keyof, ptab, coeffs4, excl_off, excl_list, \
+        index_of_id, cell_of, O, queue
+
+/* Table mode.  The occupancy hint keeps it at 4 waves per SIMD (128
+   VGPRs); the kernel is bound by instruction issue and latency, and 4 waves
+   measured faster than 3 without the hint. */
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbond(NB_PARAMS) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const struct mlp_net N = {};
+    nonbond_body<false>(smem, NB_ARGS, N, nullptr);
+}
+
+/* Surrogate mode (engine_flag_mlp). */
+__global__ __launch_bounds__(64) void k_nonbond_mlp(NB_PARAMS, struct mlp_net N, const int *__restrict__ ptype) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    nonbond_body<true>(smem, NB_ARGS, N, ptype);
 }
 
 /* ------------------------------------------------------------ k_reduce */
@@ -817,8 +961,13 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipSetDevice(g->device);
     (void)hipStreamSynchronize(g->stream);
     gpu_free_pass(g);
+    gpu_mlp_free(g);
     (void)hipFree(g->d_ptab);
     (void)hipFree(g->d_coeffs);
+    (void)hipFree(g->d_ptype);
+    free(g->h_tabof);
+    free(g->h_pair_ti);
+    free(g->h_pair_tj);
     (void)hipFree(g->d_nbr);
     (void)hipFree(g->d_nbr_off);
     (void)hipFree(g->d_origin);
@@ -883,16 +1032,25 @@ extern "C" int engine_gpu_load(struct engine *e) {
     g->max_type = mt;
 
     /* Tables: one device table per distinct potential, FP32-rounded, and a
-       header per type pair. */
+       header per type pair.  A distinct potential is a species pair: its
+       index (first appearance in type-pair order) is the pair id that the
+       MLP surrogate uses. */
     {
         struct potential **uniq = (struct potential **)calloc((size_t)mt * mt, sizeof(*uniq));
         int *tabof = (int *)malloc(sizeof(int) * mt * mt);
         int *chunk0 = (int *)malloc(sizeof(int) * mt * mt);
-        for (k = 0; k < mt * mt; k++) {
+        float4 *ptab = (float4 *)malloc(sizeof(float4) * mt * mt);
+        float *coef = NULL;
+        int bad = 0;
+        if (uniq == NULL || tabof == NULL || chunk0 == NULL || ptab == NULL) bad = engine_err_malloc;
+        for (k = 0; !bad && k < mt * mt; k++) {
             struct potential *p = e->p[k];
             tabof[k] = -1;
             if (p == NULL) continue;
-            if (p->n < 1 || p->n > 511) return gerror_(engine_err_potential, "table with more than 511 intervals");
+            if (p->n < 1 || p->n > 511) {
+                bad = engine_err_potential;
+                break;
+            }
             for (j = 0; j < ntab; j++)
                 if (uniq[j] == p) break;
             if (j == ntab) {
@@ -902,35 +1060,56 @@ extern "C" int engine_gpu_load(struct engine *e) {
             }
             tabof[k] = j;
         }
-        if (nchunks >= (1 << 22)) return gerror_(engine_err_potential, "tables too large");
-        float4 *ptab = (float4 *)malloc(sizeof(float4) * mt * mt);
-        float *coef = (float *)malloc(sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1));
-        for (j = 0; j < ntab; j++) {
-            struct potential *p = uniq[j];
-            for (k = 0; k < (p->n + 1) * potential_chunk; k++) coef[8 * chunk0[j] + k] = (float)p->c[k];
-        }
-        for (k = 0; k < mt * mt; k++) {
-            if (tabof[k] < 0) {
-                int m1 = -1;
-                ptab[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                memcpy(&ptab[k].w, &m1, sizeof(int));
-            } else {
-                struct potential *p = uniq[tabof[k]];
-                unsigned int w = (unsigned int)p->n | ((unsigned int)chunk0[tabof[k]] << 9);
-                ptab[k] = make_float4((float)p->alpha[0], (float)p->alpha[1], (float)p->alpha[2], 0.f);
-                memcpy(&ptab[k].w, &w, sizeof(unsigned int));
+        if (!bad && nchunks >= (1 << 22)) bad = engine_err_potential;
+        if (!bad && (coef = (float *)malloc(sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1))) == NULL)
+            bad = engine_err_malloc;
+        if (!bad) {
+            for (j = 0; j < ntab; j++) {
+                struct potential *p = uniq[j];
+                for (k = 0; k < (p->n + 1) * potential_chunk; k++) coef[8 * chunk0[j] + k] = (float)p->c[k];
             }
+            for (k = 0; k < mt * mt; k++) {
+                if (tabof[k] < 0) {
+                    int m1 = -1;
+                    ptab[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    memcpy(&ptab[k].w, &m1, sizeof(int));
+                } else {
+                    struct potential *p = uniq[tabof[k]];
+                    unsigned int w = (unsigned int)p->n | ((unsigned int)chunk0[tabof[k]] << 9);
+                    ptab[k] = make_float4((float)p->alpha[0], (float)p->alpha[1], (float)p->alpha[2], 0.f);
+                    memcpy(&ptab[k].w, &w, sizeof(unsigned int));
+                }
+            }
+            g->ntab = ntab;
+            g->h_tabof = tabof;
+            g->h_pair_ti = (int *)malloc(sizeof(int) * (ntab > 0 ? ntab : 1));
+            g->h_pair_tj = (int *)malloc(sizeof(int) * (ntab > 0 ? ntab : 1));
+            if (g->h_pair_ti == NULL || g->h_pair_tj == NULL) bad = engine_err_malloc;
+            for (k = mt * mt - 1; !bad && k >= 0; k--)
+                if (tabof[k] >= 0 && k / mt <= k % mt) {
+                    g->h_pair_ti[tabof[k]] = k / mt;
+                    g->h_pair_tj[tabof[k]] = k % mt;
+                }
         }
-        g->ntab = ntab;
-        HIPCHECK(hipMalloc(&g->d_ptab, sizeof(float4) * mt * mt));
-        HIPCHECK(hipMalloc(&g->d_coeffs, sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1)));
-        HIPCHECK(hipMemcpy(g->d_ptab, ptab, sizeof(float4) * mt * mt, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(g->d_coeffs, coef, sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1), hipMemcpyHostToDevice));
+        if (!bad) {
+            if (hipMalloc(&g->d_ptab, sizeof(float4) * mt * mt) != hipSuccess ||
+                hipMalloc(&g->d_coeffs, sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1)) != hipSuccess ||
+                hipMalloc(&g->d_ptype, sizeof(int) * mt * mt) != hipSuccess ||
+                hipMemcpy(g->d_ptab, ptab, sizeof(float4) * mt * mt, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(g->d_coeffs, coef, sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1),
+                          hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(g->d_ptype, tabof, sizeof(int) * mt * mt, hipMemcpyHostToDevice) != hipSuccess)
+                bad = engine_err_gpu;
+        }
         free(uniq);
-        free(tabof);
         free(chunk0);
         free(ptab);
         free(coef);
+        if (bad) {
+            if (g->h_tabof != tabof) free(tabof);
+            return gerror_(bad, bad == engine_err_potential ? "table with more than 511 intervals or too large"
+                                                            : "table upload failed");
+        }
     }
 
     /* Geometry constants of the sorted window (runner_dopair.c:424-428). */
@@ -1081,7 +1260,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
         HIPCHECK(hipGetDeviceProperties(&prop, g->device));
         g->ncu = prop.multiProcessorCount;
     }
-    g->lds_nb = 0;
+    g->lds_nb[0] = g->lds_nb[1] = 0;
     return engine_err_ok;
 }
 
@@ -1134,7 +1313,9 @@ static int gpu_upload(struct engine *e) {
 }
 
 /* Launch the kernels of one pass; optional events around each. */
-static int gpu_launch(struct gpu_state *g, int timed) {
+/* One device pass on the resident particles; mlp selects the surrogate
+   mode of the pair kernel. */
+static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     struct nb_params P = g->P;
     int nb;
     size_t lds = 0;
@@ -1153,16 +1334,23 @@ static int gpu_launch(struct gpu_state *g, int timed) {
     P.lds_stage = (int)lds;
     lds += sizeof(float4) * maxn;
     P.lds_hit = (int)lds;
-    lds += sizeof(unsigned short) * 64 * MD_HCAP;
+    lds += sizeof(unsigned short) * 64 * (mlp ? MD_HCAP_MLP : MD_HCAP);
+    lds = (lds + 15) & ~(size_t)15;
+    P.lds_dense = (int)lds;
+    if (mlp) lds += 16 * 64 * MD_HCAP_MLP + 4 * 64 * MD_XCAP_MLP;
     if (lds > 64 * 1024) return gerror_(engine_err_maxparts, "LDS staging exceeds 64 KiB");
-    if (lds != g->lds_nb) {
+    if (lds != g->lds_nb[mlp]) {
         /* Persistent grid: as many single-wave blocks as can be resident. */
         int per_cu = 0;
-        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond, 64, lds));
+        if (mlp)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_mlp, 64, lds));
+        else
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond, 64, lds));
         if (per_cu < 1) per_cu = 1;
-        g->grid_nb = g->ncu * per_cu;
-        g->lds_nb = lds;
+        g->grid_nb[mlp] = g->ncu * per_cu;
+        g->lds_nb[mlp] = lds;
     }
+    if (mlp && g->mlp_np == 0) return gerror_(engine_err_range, "engine_flag_mlp without engine_mlp_init");
 
     if (timed) HIPCHECK(hipEventRecord(g->ev[0], g->stream));
     nb = (g->nids + 255) / 256;
@@ -1179,10 +1367,16 @@ static int gpu_launch(struct gpu_state *g, int timed) {
     O.flo = g->d_flo;
     O.cell_acc = g->d_cell_acc;
     O.cell_cnt = g->d_cell_cnt;
-    hipLaunchKernelGGL(k_nonbond, dim3(g->grid_nb), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
-                       g->d_nbr_off, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist, g->d_keyof,
-                       g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list, g->d_index_of_id,
-                       g->d_cell_of, O, g->d_queue);
+    if (mlp)
+        hipLaunchKernelGGL(k_nonbond_mlp, dim3(g->grid_nb[1]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
+                           g->d_nbr_off, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
+                           g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
+                           g->d_index_of_id, g->d_cell_of, O, g->d_queue, gpu_mlp_net(g), g->d_ptype);
+    else
+        hipLaunchKernelGGL(k_nonbond, dim3(g->grid_nb[0]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
+                           g->d_nbr_off, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
+                           g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
+                           g->d_index_of_id, g->d_cell_of, O, g->d_queue);
     if (timed) HIPCHECK(hipEventRecord(g->ev[2], g->stream));
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, g->stream, g->ncells, g->d_cell_acc, g->d_cell_cnt, g->d_tot,
                        g->d_tot_cnt);
@@ -1244,7 +1438,7 @@ extern "C" int engine_nonbond_gpu(struct engine *e) {
     HIPCHECK(hipSetDevice(g->device));
     if (gpu_upload(e) < 0) return engine_err_gpu;
     HIPCHECK(hipMemsetAsync(g->d_queue + MD_NQUEUE, 0, sizeof(int), g->stream));
-    if (gpu_launch(g, 1) < 0) return engine_err_gpu;
+    if (gpu_launch(g, 1, (e->flags & engine_flag_mlp) != 0) < 0) return engine_err_gpu;
     HIPCHECK(hipMemcpyAsync(g->h_fhi, g->d_fhi, sizeof(float4) * g->nparts, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipMemcpyAsync(g->h_flo, g->d_flo, sizeof(float4) * g->nparts, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipMemcpyAsync(g->h_tot, g->d_tot, sizeof(double) * 3, hipMemcpyDeviceToHost, g->stream));
@@ -1286,10 +1480,10 @@ extern "C" int engine_gpu_time(struct engine *e, int nsteps, int nwarm, double *
     if (g == NULL || g->nparts_cap == 0) return gerror_(engine_err_nogpu, "no resident particles");
     HIPCHECK(hipSetDevice(g->device));
     for (k = 0; k < nwarm; k++)
-        if (gpu_launch(g, 0) < 0) return engine_err_gpu;
+        if (gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0) < 0) return engine_err_gpu;
     HIPCHECK(hipStreamSynchronize(g->stream));
     for (k = 0; k < nsteps; k++) {
-        if (gpu_launch(g, 1) < 0) return engine_err_gpu;
+        if (gpu_launch(g, 1, (e->flags & engine_flag_mlp) != 0) < 0) return engine_err_gpu;
         HIPCHECK(hipEventSynchronize(g->ev[3]));
         HIPCHECK(hipEventElapsedTime(&t, g->ev[0], g->ev[1]));
         acc[0] += t;

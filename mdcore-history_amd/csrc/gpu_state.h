@@ -1,0 +1,116 @@
+/*
+ * Device state of the gfx950 nonbonded engine, shared by the pair kernels
+ * (engine_gpu.hip) and the MLP surrogate (mlp_gpu.hip).  Internal header.
+ */
+#ifndef GPU_STATE_H
+#define GPU_STATE_H
+
+#include <hip/hip_runtime.h>
+
+#include "engine_gpu.h"
+#include "mlp_tile.h"
+
+#define MD_NQUEUE 8
+#define MD_MAX_CELL 1024 /* particle index in a packed record: 10 bits */
+#define MD_MAX_TYPES 64  /* particle type in a packed record: 6 bits */
+
+#define gerror_(id, msg) errs_register((id), (msg), __LINE__, __func__, __FILE__)
+
+#define HIPCHECK(call)                                                                            \
+    do {                                                                                          \
+        hipError_t err_ = (call);                                                                 \
+        if (err_ != hipSuccess) {                                                                 \
+            errs_register(engine_err_gpu, hipGetErrorString(err_), __LINE__, __func__, __FILE__); \
+            return engine_err_gpu;                                                                \
+        }                                                                                         \
+    } while (0)
+
+/* ------------------------------------------------------------ parameters */
+
+struct nb_params {
+    int ncells, nparts, nids, max_type, proj;
+    int dmaxdist;
+    double dscale, bias, cutoff2;
+    double dim[3], h[3];
+    float rc2f, band;
+    int maxn, maxnbr;
+    /* dynamic LDS layout (byte offsets) */
+    int lds_own_key, lds_kfl, lds_nbe, lds_nbc, lds_stage, lds_hit, lds_dense;
+};
+
+/* Neighbour-list entry of an owner cell: x = other cell, y = sid | orient<<4
+   | self<<5 (orient 0: owner is the task's i cell), z = dnshift with the
+   reference window |shift|, w = dnshift with the projected shift.n.
+   Type-pair table header ptab[ti*mt+tj] = (alpha0, alpha1, alpha2,
+   bits(n | chunk0 << 9)) or w = bits(-1) when the pair has no potential. */
+
+struct gpu_state {
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev[8];
+    int ncells, ntab, max_type, nparts_cap, nids_cap, maxcount, maxnbr;
+    int nparts, nids;
+    struct nb_params P;
+    int grid_nb[2], ncu; /* [0] table mode, [1] surrogate mode */
+    size_t lds_nb[2];
+    /* static */
+    float4 *d_ptab;
+    float *d_coeffs;
+    int4 *d_nbr;
+    int *d_nbr_off;
+    double *d_origin;
+    int *d_loc;
+    unsigned int *d_sortflags;
+    int *d_excl_off, *d_excl_list;
+    /* per pass */
+    int2 *d_cnt_off;
+    int *d_count, *d_offset;
+    double *d_x;
+    int *d_type, *d_id, *d_index_of_id, *d_cell_of;
+    float4 *d_xf, *d_fhi, *d_flo;
+    unsigned int *d_sortlist;
+    unsigned short *d_keyof;
+    double *d_cell_acc, *d_tot;
+    unsigned long long *d_cell_cnt, *d_tot_cnt;
+    int *d_queue;
+    /* pinned host staging */
+    double *h_x;
+    int *h_type, *h_id, *h_count, *h_offset;
+    int2 *h_cnt_off;
+    float4 *h_fhi, *h_flo;
+    double *h_tot;
+    unsigned long long *h_tot_cnt;
+    /* species pairs = distinct potentials, in table order */
+    int *h_tabof;      /* [mt*mt] type pair -> pair id or -1 */
+    int *h_pair_ti, *h_pair_tj;
+    int *d_ptype;      /* [mt*mt] type pair -> pair id or -1 */
+    /* MLP surrogate (mlp_gpu.hip); mlp_np == 0 until engine_mlp_init */
+    int mlp_np, mlp_nparams;
+    long long mlp_step;
+    float *d_theta, *d_grad, *d_adam_m, *d_adam_v, *d_gb1e, *d_b1e, *d_loss;
+    mlp_bf16x8 *d_w2f, *d_w2b;
+    float4 *d_msc;
+    float *d_sr, *d_sE, *d_sF;
+    int *d_sp, *d_tiles;
+    long long d_samples_cap;
+};
+
+/* Device view of the published network of g. */
+static inline struct mlp_net gpu_mlp_net(const struct gpu_state *g) {
+    struct mlp_net N;
+    N.w2f = g->d_w2f;
+    N.w2b = g->d_w2b;
+    N.w1 = g->d_theta + MLP_OFF_W1;
+    N.b1e = g->d_b1e;
+    N.b2 = g->d_theta + MLP_OFF_B2;
+    N.w3 = g->d_theta + MLP_OFF_W3(g->mlp_np);
+    N.b3 = g->d_theta + MLP_OFF_B3(g->mlp_np);
+    N.sc = g->d_msc;
+    return N;
+}
+
+void gpu_mlp_free(struct gpu_state *g);
+
+#endif /* GPU_STATE_H */
+
+

@@ -1,0 +1,649 @@
+/*
+ * MLP surrogate of the interpolated pair potential (no reference
+ * counterpart; the reference evaluates only the tables of
+ * potential_eval.h:67-104, which stay the judge and the fallback here).
+ *
+ * Training runs on the GPU with this file's own kernels:
+ *   k_mlp_grad     forward + backward of tiles of 16 samples of one species
+ *                  pair.  Layer 2 and both of its backward products
+ *                  (dH1 = W2^T dZ2 and dW2 = dZ2 H1^T) run on
+ *                  v_mfma_f32_16x16x32_bf16 with bf16 operands and fp32
+ *                  accumulation; the rest is fp32 VALU.
+ *   k_mlp_chain    gradient of b1, E1 and the embeddings from the gradient
+ *                  of the per-pair bias b1e = b1 + E1 emb[p];
+ *   k_mlp_adam     Adam on the fp32 master parameters;
+ *   k_mlp_publish  derived device operands: b1e and the bf16 W2 operands in
+ *                  the lane order of the MFMA tiles.
+ * k_mlp_eval evaluates the network on arbitrary (r, pair) samples.
+ * The forward pass and the lane layout are in mlp_tile.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gpu_state.h"
+
+/* ------------------------------------------------------------ kernels */
+
+__global__ __launch_bounds__(256) void k_mlp_fill(float *__restrict__ a, long long n, float v) {
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        a[k] = v;
+}
+
+/* K index carried by element e of lane group g in K-block kb. */
+__device__ __forceinline__ int mlp_kperm(int kb, int g, int e) { return 16 * (2 * kb + (e >> 2)) + 4 * g + (e & 3); }
+
+__global__ __launch_bounds__(256) void k_mlp_publish(const float *__restrict__ theta, int np, float *__restrict__ b1e,
+                                                     mlp_bf16x8 *__restrict__ w2f, mlp_bf16x8 *__restrict__ w2b) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 512) {
+        const int blk = t >> 6, l = t & 63, mb = blk >> 1, kb = blk & 1, g = l >> 4;
+        const int row = 16 * mb + (l & 15);
+        mlp_bf16x8 a, b;
+        for (int e = 0; e < 8; e++) {
+            const int k = mlp_kperm(kb, g, e);
+            a[e] = (__bf16)theta[MLP_OFF_W2 + row * MLP_H + k]; /* W2[out = row][in = k]   */
+            b[e] = (__bf16)theta[MLP_OFF_W2 + k * MLP_H + row]; /* W2^T: [in = row][out = k] */
+        }
+        w2f[t] = a;
+        w2b[t] = b;
+    }
+    if (t < np * MLP_H) {
+        const int p = t / MLP_H, u = t % MLP_H;
+        float acc = theta[MLP_OFF_B1 + u];
+        for (int e = 0; e < MLP_EMB; e++) acc += theta[MLP_OFF_E1 + u * MLP_EMB + e] * theta[MLP_OFF_EMB + p * MLP_EMB + e];
+        b1e[t] = acc;
+    }
+}
+
+/* Sum over the 16 pair columns of a lane group. */
+__device__ __forceinline__ float mlp_sum16(float v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    return v;
+}
+
+/* One wave per block; each block walks pairs of tiles (32 samples, the K of
+   the dW2 product).  Samples of tile t are s = 16 t + n, n = 0..15, and all
+   16 share species pair sp[16 t].  Loss per sample:
+   wE (y0 - asinh(E/sE))^2 + wF (y1 - asinh(F/sF))^2; gradients are of the
+   sum times inv_n. */
+__global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *__restrict__ sr,
+                                                 const int *__restrict__ sp, const float *__restrict__ tE,
+                                                 const float *__restrict__ tF, const int *__restrict__ tiles,
+                                                 int ntiles, float wE, float wF, float inv_n, int np,
+                                                 float *__restrict__ grad, float *__restrict__ gb1e,
+                                                 float *__restrict__ loss) {
+    __shared__ __attribute__((aligned(16))) __bf16 h1t[MLP_H][32]; /* [in unit][sample of the tile pair] */
+    __shared__ __attribute__((aligned(16))) __bf16 dzt[MLP_H][32]; /* [out unit][sample]               */
+    const int lane = threadIdx.x, n = lane & 15, g = lane >> 4;
+    mlp_f4 dw2[16];
+    float dw1a[16], dw1b[16], db2[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        dw2[q] = (mlp_f4){0.f, 0.f, 0.f, 0.f};
+        dw1a[q] = dw1b[q] = db2[q] = 0.f;
+    }
+    float lsum = 0.f;
+    const int ntp = (ntiles + 1) / 2;
+    for (int tp = blockIdx.x; tp < ntp; tp += gridDim.x) {
+        for (int sub = 0; sub < 2; sub++) {
+            const int ti = 2 * tp + sub;
+            const bool valid = ti < ntiles;
+            const int tile = tiles[valid ? ti : 0];
+            const int s = 16 * tile + n;
+            const int p = sp[16 * tile];
+            const float r = sr[s];
+            float z1[16], z2[16], h2[16], y0, y1;
+            mlp_bf16x8 h1b[2];
+            mlp_forward16(N, r, p, lane, z1, h1b, z2, h2, y0, y1);
+            const float4 sc = N.sc[p];
+            const float d0 = y0 - asinhf(tE[s] / sc.z), d1 = y1 - asinhf(tF[s] / sc.w);
+            if (valid && g == 0) lsum += wE * d0 * d0 + wF * d1 * d1;
+            const float dy0 = valid ? 2.0f * wE * d0 * inv_n : 0.0f;
+            const float dy1 = valid ? 2.0f * wF * d1 * inv_n : 0.0f;
+
+            /* Heads: dW3[p], db3[p]; dZ2. */
+            const float *w3 = N.w3 + (size_t)p * 2 * MLP_H;
+            float dz2[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int u = mlp_unit(j, lane);
+                const float dh2 = dy0 * w3[u] + dy1 * w3[MLP_H + u];
+                dz2[j] = dh2 * mlp_dsilu(z2[j]);
+                db2[j] += dz2[j];
+                const float ga = mlp_sum16(dy0 * h2[j]);
+                const float gb = mlp_sum16(dy1 * h2[j]);
+                if (valid && n == 0) {
+                    atomicAdd(&grad[MLP_OFF_W3(np) + p * 2 * MLP_H + u], ga);
+                    atomicAdd(&grad[MLP_OFF_W3(np) + p * 2 * MLP_H + MLP_H + u], gb);
+                }
+            }
+            {
+                const float ga = mlp_sum16(dy0), gb = mlp_sum16(dy1);
+                if (valid && lane == 0) {
+                    atomicAdd(&grad[MLP_OFF_B3(np) + 2 * p], ga);
+                    atomicAdd(&grad[MLP_OFF_B3(np) + 2 * p + 1], gb);
+                }
+            }
+
+            /* dH1 = W2^T dZ2 on MFMA; dZ1. */
+            mlp_bf16x8 dzb[2];
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) dzb[kb][e] = (__bf16)dz2[8 * kb + e];
+            float rn, irn;
+            mlp_inputs(sc, r, rn, irn);
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) {
+                mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2b[(mb * 2 + 0) * 64 + lane], dzb[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2b[(mb * 2 + 1) * 64 + lane], dzb[1], acc, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int j = 4 * mb + i;
+                    const float dz1 = acc[i] * mlp_dsilu(z1[j]);
+                    dw1a[j] += dz1 * rn;
+                    dw1b[j] += dz1 * irn;
+                    const float gsum = mlp_sum16(dz1);
+                    if (valid && n == 0) atomicAdd(&gb1e[p * MLP_H + mlp_unit(j, lane)], gsum);
+                }
+            }
+
+            /* Stage H1 and dZ2 (bf16) for dW2. */
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int u = mlp_unit(j, lane);
+                h1t[u][16 * sub + n] = h1b[j >> 3][j & 7];
+                dzt[u][16 * sub + n] = dzb[j >> 3][j & 7];
+            }
+        }
+        __syncthreads();
+        /* dW2[m][k] += sum over the 32 samples of dZ2[m] H1[k]. */
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            const mlp_bf16x8 a = *(const mlp_bf16x8 *)&dzt[16 * mb + n][8 * g];
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++) {
+                const mlp_bf16x8 b = *(const mlp_bf16x8 *)&h1t[16 * kb + n][8 * g];
+                dw2[mb * 4 + kb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, dw2[mb * 4 + kb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    /* Flush this wave's sums. */
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                atomicAdd(&grad[MLP_OFF_W2 + (16 * mb + 4 * g + i) * MLP_H + 16 * kb + n], dw2[mb * 4 + kb][i]);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int u = mlp_unit(j, lane);
+        const float a = mlp_sum16(dw1a[j]), b = mlp_sum16(dw1b[j]), c = mlp_sum16(db2[j]);
+        if (n == 0) {
+            atomicAdd(&grad[MLP_OFF_W1 + 2 * u], a);
+            atomicAdd(&grad[MLP_OFF_W1 + 2 * u + 1], b);
+            atomicAdd(&grad[MLP_OFF_B2 + u], c);
+        }
+    }
+    lsum = mlp_sum16(lsum);
+    if (lane == 0) atomicAdd(loss, lsum * inv_n);
+}
+
+/* b1e[p][u] = b1[u] + sum_e E1[u][e] emb[p][e]: gradients of b1, E1, emb. */
+__global__ __launch_bounds__(256) void k_mlp_chain(const float *__restrict__ theta, int np,
+                                                   const float *__restrict__ gb1e, float *__restrict__ grad) {
+    const int t = threadIdx.x;
+    if (t < MLP_H) {
+        float b = 0.f, e1[MLP_EMB];
+        for (int e = 0; e < MLP_EMB; e++) e1[e] = 0.f;
+        for (int p = 0; p < np; p++) {
+            const float gg = gb1e[p * MLP_H + t];
+            b += gg;
+            for (int e = 0; e < MLP_EMB; e++) e1[e] += gg * theta[MLP_OFF_EMB + p * MLP_EMB + e];
+        }
+        grad[MLP_OFF_B1 + t] += b;
+        for (int e = 0; e < MLP_EMB; e++) grad[MLP_OFF_E1 + t * MLP_EMB + e] += e1[e];
+    }
+    for (int q = t; q < np * MLP_EMB; q += blockDim.x) {
+        const int p = q / MLP_EMB, e = q % MLP_EMB;
+        float acc = 0.f;
+        for (int u = 0; u < MLP_H; u++) acc += gb1e[p * MLP_H + u] * theta[MLP_OFF_E1 + u * MLP_EMB + e];
+        grad[MLP_OFF_EMB + q] += acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ theta, const float *__restrict__ grad,
+                                                  float *__restrict__ m, float *__restrict__ v, int n, float lr,
+                                                  float b1, float b2, float c1, float c2, float eps) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float gk = grad[k];
+    const float mk = b1 * m[k] + (1.0f - b1) * gk;
+    const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    theta[k] -= lr * (mk * c1) / (sqrtf(vk * c2) + eps);
+}
+
+__global__ __launch_bounds__(64) void k_mlp_eval(struct mlp_net N, const float *__restrict__ r,
+                                                 const int *__restrict__ pair, long long n, float *__restrict__ E,
+                                                 float *__restrict__ F) {
+    const int lane = threadIdx.x;
+    for (long long tile = blockIdx.x; 16 * tile < n; tile += gridDim.x) {
+        const long long s = 16 * tile + (lane & 15);
+        const long long sv = s < n ? s : n - 1;
+        const int p = pair[sv];
+        const float rr = r[sv];
+        float z1[16], z2[16], h2[16], y0, y1;
+        mlp_bf16x8 h1b[2];
+        mlp_forward16(N, rr, p, lane, z1, h1b, z2, h2, y0, y1);
+        if (lane < 16 && s < n) {
+            float e, f;
+            mlp_energy_force(N.sc[p], rr, y0, y1, e, f);
+            E[s] = e;
+            F[s] = f * rr;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ host side */
+
+static unsigned long long mlp_rng(unsigned long long *s) {
+    unsigned long long z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double mlp_normal(unsigned long long *s) {
+    double u1 = ((mlp_rng(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    double u2 = ((mlp_rng(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+void gpu_mlp_free(struct gpu_state *g) {
+    (void)hipFree(g->d_theta);
+    (void)hipFree(g->d_grad);
+    (void)hipFree(g->d_adam_m);
+    (void)hipFree(g->d_adam_v);
+    (void)hipFree(g->d_gb1e);
+    (void)hipFree(g->d_b1e);
+    (void)hipFree(g->d_loss);
+    (void)hipFree(g->d_w2f);
+    (void)hipFree(g->d_w2b);
+    (void)hipFree(g->d_msc);
+    (void)hipFree(g->d_sr);
+    (void)hipFree(g->d_sp);
+    (void)hipFree(g->d_sE);
+    (void)hipFree(g->d_sF);
+    (void)hipFree(g->d_tiles);
+    g->d_theta = g->d_grad = g->d_adam_m = g->d_adam_v = g->d_gb1e = g->d_b1e = g->d_loss = NULL;
+    g->d_w2f = g->d_w2b = NULL;
+    g->d_msc = NULL;
+    g->d_sr = g->d_sE = g->d_sF = NULL;
+    g->d_sp = g->d_tiles = NULL;
+    g->d_samples_cap = 0;
+    g->mlp_np = g->mlp_nparams = 0;
+    g->mlp_step = 0;
+}
+
+static struct gpu_state *mlp_state(struct engine *e, int need_net) {
+    struct gpu_state *g;
+    if (e == NULL) {
+        gerror_(engine_err_null, "NULL engine");
+        return NULL;
+    }
+    g = (struct gpu_state *)e->gpu;
+    if (g == NULL) {
+        gerror_(engine_err_nogpu, "engine_gpu_load was not called");
+        return NULL;
+    }
+    if (need_net && g->mlp_np == 0) {
+        gerror_(engine_err_range, "engine_mlp_init was not called");
+        return NULL;
+    }
+    if (hipSetDevice(g->device) != hipSuccess) {
+        gerror_(engine_err_gpu, "hipSetDevice failed");
+        return NULL;
+    }
+    return g;
+}
+
+static int mlp_publish(struct gpu_state *g) {
+    const int nthr = g->mlp_np * MLP_H > 512 ? g->mlp_np * MLP_H : 512;
+    hipLaunchKernelGGL(k_mlp_publish, dim3((nthr + 255) / 256), dim3(256), 0, g->stream, g->d_theta, g->mlp_np,
+                       g->d_b1e, g->d_w2f, g->d_w2b);
+    HIPCHECK(hipGetLastError());
+    return engine_err_ok;
+}
+
+/* Grow the device sample buffers to hold n samples (and n/16 tile ids). */
+static int mlp_samples(struct gpu_state *g, long long n) {
+    if (n <= g->d_samples_cap) return engine_err_ok;
+    (void)hipFree(g->d_sr);
+    (void)hipFree(g->d_sp);
+    (void)hipFree(g->d_sE);
+    (void)hipFree(g->d_sF);
+    (void)hipFree(g->d_tiles);
+    g->d_samples_cap = 0;
+    HIPCHECK(hipMalloc(&g->d_sr, sizeof(float) * n));
+    HIPCHECK(hipMalloc(&g->d_sp, sizeof(int) * n));
+    HIPCHECK(hipMalloc(&g->d_sE, sizeof(float) * n));
+    HIPCHECK(hipMalloc(&g->d_sF, sizeof(float) * n));
+    HIPCHECK(hipMalloc(&g->d_tiles, sizeof(int) * (n / 16 + 1)));
+    g->d_samples_cap = n;
+    return engine_err_ok;
+}
+
+extern "C" int engine_mlp_pairs(struct engine *e, int *ti, int *tj, int max) {
+    struct gpu_state *g = mlp_state(e, 0);
+    int k;
+    if (g == NULL) return engine_err;
+    for (k = 0; k < g->ntab && k < max; k++) {
+        if (ti) ti[k] = g->h_pair_ti[k];
+        if (tj) tj[k] = g->h_pair_tj[k];
+    }
+    return g->ntab;
+}
+
+extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const double *r_hi, const double *escale,
+                               const double *fscale, unsigned long long seed) {
+    struct gpu_state *g = mlp_state(e, 0);
+    int np, nparams, k;
+    float *theta;
+    float4 *sc;
+    if (g == NULL) return engine_err;
+    if (r_lo == NULL || r_hi == NULL || escale == NULL || fscale == NULL)
+        return gerror_(engine_err_null, "NULL pair scales");
+    np = g->ntab;
+    if (np < 1) return gerror_(engine_err_range, "no potentials");
+    for (k = 0; k < np; k++)
+        if (!(r_lo[k] > 0.0 && r_hi[k] > r_lo[k] && escale[k] > 0.0 && fscale[k] > 0.0))
+            return gerror_(engine_err_range, "pair scales must satisfy 0 < r_lo < r_hi, scales > 0");
+    gpu_mlp_free(g);
+    nparams = MLP_NPARAMS(np);
+    theta = (float *)malloc(sizeof(float) * nparams);
+    sc = (float4 *)malloc(sizeof(float4) * np);
+    if (theta == NULL || sc == NULL) {
+        free(theta);
+        free(sc);
+        return gerror_(engine_err_malloc, "malloc failed");
+    }
+    /* Initial weights: normal, scaled by fan-in (SiLU gain for the hidden layers). */
+    {
+        unsigned long long s = seed;
+        for (k = 0; k < nparams; k++) theta[k] = 0.0f;
+        for (k = 0; k < 2 * MLP_H; k++) theta[MLP_OFF_W1 + k] = (float)(mlp_normal(&s) * 1.0);
+        for (k = 0; k < MLP_H * MLP_EMB; k++) theta[MLP_OFF_E1 + k] = (float)(mlp_normal(&s) * sqrt(1.0 / MLP_EMB));
+        for (k = 0; k < MLP_H * MLP_H; k++) theta[MLP_OFF_W2 + k] = (float)(mlp_normal(&s) * sqrt(2.0 / MLP_H));
+        for (k = 0; k < MLP_EMB * np; k++) theta[MLP_OFF_EMB + k] = (float)mlp_normal(&s);
+        for (k = 0; k < 2 * MLP_H * np; k++) theta[MLP_OFF_W3(np) + k] = (float)(mlp_normal(&s) * sqrt(1.0 / MLP_H));
+    }
+    for (k = 0; k < np; k++) sc[k] = make_float4((float)r_lo[k], (float)r_hi[k], (float)escale[k], (float)fscale[k]);
+    g->mlp_np = np;
+    g->mlp_nparams = nparams;
+    g->mlp_step = 0;
+    if (hipMalloc(&g->d_theta, sizeof(float) * nparams) != hipSuccess ||
+        hipMalloc(&g->d_grad, sizeof(float) * nparams) != hipSuccess ||
+        hipMalloc(&g->d_adam_m, sizeof(float) * nparams) != hipSuccess ||
+        hipMalloc(&g->d_adam_v, sizeof(float) * nparams) != hipSuccess ||
+        hipMalloc(&g->d_gb1e, sizeof(float) * np * MLP_H) != hipSuccess ||
+        hipMalloc(&g->d_b1e, sizeof(float) * np * MLP_H) != hipSuccess ||
+        hipMalloc(&g->d_w2f, sizeof(mlp_bf16x8) * 512) != hipSuccess ||
+        hipMalloc(&g->d_w2b, sizeof(mlp_bf16x8) * 512) != hipSuccess ||
+        hipMalloc(&g->d_msc, sizeof(float4) * np) != hipSuccess ||
+        hipMemcpy(g->d_theta, theta, sizeof(float) * nparams, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_msc, sc, sizeof(float4) * np, hipMemcpyHostToDevice) != hipSuccess) {
+        free(theta);
+        free(sc);
+        gpu_mlp_free(g);
+        return gerror_(engine_err_gpu, "MLP allocation failed");
+    }
+    free(theta);
+    free(sc);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_adam_m, (long long)nparams, 0.0f);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_adam_v, (long long)nparams, 0.0f);
+    if (mlp_publish(g) < 0) return engine_err_gpu;
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    return nparams;
+}
+
+extern "C" int engine_mlp_params(struct engine *e, float *theta, int get) {
+    struct gpu_state *g = mlp_state(e, 1);
+    if (g == NULL) return engine_err;
+    if (theta == NULL) return g->mlp_nparams;
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    if (get) {
+        HIPCHECK(hipMemcpy(theta, g->d_theta, sizeof(float) * g->mlp_nparams, hipMemcpyDeviceToHost));
+    } else {
+        HIPCHECK(hipMemcpy(g->d_theta, theta, sizeof(float) * g->mlp_nparams, hipMemcpyHostToDevice));
+        if (mlp_publish(g) < 0) return engine_err_gpu;
+        HIPCHECK(hipStreamSynchronize(g->stream));
+    }
+    return g->mlp_nparams;
+}
+
+/* Check samples: pair ids in range, r > 0, finite targets. */
+static int mlp_check(const struct gpu_state *g, const float *r, const int *pair, long long n) {
+    for (long long k = 0; k < n; k++)
+        if (pair[k] < 0 || pair[k] >= g->mlp_np || !(r[k] > 0.0f))
+            return gerror_(engine_err_range, "sample with pair id out of range or r <= 0");
+    return engine_err_ok;
+}
+
+/* One gradient evaluation over the tiles tiles[0..ntiles) (device array) of
+   the uploaded samples: zero the gradients, k_mlp_grad, k_mlp_chain.  The
+   mean loss adds into *loss (device). */
+static int mlp_gradient(struct gpu_state *g, const int *tiles, int ntiles, float wE, float wF, float *loss) {
+    const int np = g->mlp_np;
+    const int ntp = (ntiles + 1) / 2;
+    const int grid = ntp < 2048 ? ntp : 2048;
+    if (ntiles < 1) return gerror_(engine_err_range, "empty batch");
+    hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_grad, (long long)g->mlp_nparams, 0.0f);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(16), dim3(256), 0, g->stream, g->d_gb1e, (long long)np * MLP_H, 0.0f);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64), 0, g->stream, gpu_mlp_net(g), g->d_sr, g->d_sp, g->d_sE,
+                       g->d_sF, tiles, ntiles, wE, wF, 1.0f / (16.0f * ntiles), np, g->d_grad, g->d_gb1e, loss);
+    hipLaunchKernelGGL(k_mlp_chain, dim3(1), dim3(256), 0, g->stream, g->d_theta, np, g->d_gb1e, g->d_grad);
+    HIPCHECK(hipGetLastError());
+    return engine_err_ok;
+}
+
+extern "C" int engine_mlp_grad(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
+                               long long n, double wE, double wF, float *grad, double *loss) {
+    struct gpu_state *g = mlp_state(e, 1);
+    long long k;
+    int *tiles;
+    float lossf = 0.0f;
+    if (g == NULL) return engine_err;
+    if (r == NULL || pair == NULL || E == NULL || F == NULL || grad == NULL) return gerror_(engine_err_null, "NULL");
+    if (n < 16 || n % 16 != 0 || n / 16 > (1 << 24)) return gerror_(engine_err_range, "n must be a positive multiple of 16");
+    if (mlp_check(g, r, pair, n) < 0) return engine_err_range;
+    for (k = 0; k < n; k++)
+        if (pair[k] != pair[k - k % 16]) return gerror_(engine_err_range, "each tile of 16 samples must share a pair id");
+    if (mlp_samples(g, n) < 0) return engine_err_gpu;
+    tiles = (int *)malloc(sizeof(int) * (n / 16));
+    if (tiles == NULL) return gerror_(engine_err_malloc, "malloc failed");
+    for (k = 0; k < n / 16; k++) tiles[k] = (int)k;
+    (void)hipFree(g->d_loss);
+    g->d_loss = NULL;
+    if (hipMalloc(&g->d_loss, sizeof(float)) != hipSuccess ||
+        hipMemcpy(g->d_sr, r, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_sp, pair, sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_sE, E, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_sF, F, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_tiles, tiles, sizeof(int) * (n / 16), hipMemcpyHostToDevice) != hipSuccess) {
+        free(tiles);
+        return gerror_(engine_err_gpu, "sample upload failed");
+    }
+    free(tiles);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(1), dim3(64), 0, g->stream, g->d_loss, 1LL, 0.0f);
+    if (mlp_gradient(g, g->d_tiles, (int)(n / 16), (float)wE, (float)wF, g->d_loss) < 0) return engine_err_gpu;
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    HIPCHECK(hipMemcpy(grad, g->d_grad, sizeof(float) * g->mlp_nparams, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(&lossf, g->d_loss, sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) *loss = lossf;
+    return engine_err_ok;
+}
+
+extern "C" int engine_mlp_train(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
+                                long long n, int steps, int batch_tiles, double lr0, double lr1, double wE, double wF,
+                                unsigned long long seed, float *loss_hist) {
+    struct gpu_state *g = mlp_state(e, 1);
+    long long k, ntiles = 0, m;
+    long long *cnt, *start;
+    float *hr, *hE, *hF;
+    int *hp, *tiles;
+    int np, step, pos;
+    unsigned long long rs = seed;
+    const float b1 = 0.9f, b2 = 0.999f;
+    if (g == NULL) return engine_err;
+    if (r == NULL || pair == NULL || E == NULL || F == NULL) return gerror_(engine_err_null, "NULL samples");
+    if (n < 1 || steps < 1 || batch_tiles < 1) return gerror_(engine_err_range, "n, steps and batch must be positive");
+    if (mlp_check(g, r, pair, n) < 0) return engine_err_range;
+    np = g->mlp_np;
+    /* Group the samples by pair into tiles of 16; a pair's last tile is
+       padded with its own first samples. */
+    cnt = (long long *)calloc(np + 1, sizeof(long long));
+    start = (long long *)calloc(np + 1, sizeof(long long));
+    if (cnt == NULL || start == NULL) {
+        free(cnt);
+        free(start);
+        return gerror_(engine_err_malloc, "malloc failed");
+    }
+    for (k = 0; k < n; k++) cnt[pair[k]]++;
+    for (k = 0; k < np; k++) ntiles += (cnt[k] + 15) / 16;
+    if (ntiles > (1 << 26)) {
+        free(cnt);
+        free(start);
+        return gerror_(engine_err_range, "too many samples");
+    }
+    m = 16 * ntiles;
+    hr = (float *)malloc(sizeof(float) * m);
+    hE = (float *)malloc(sizeof(float) * m);
+    hF = (float *)malloc(sizeof(float) * m);
+    hp = (int *)malloc(sizeof(int) * m);
+    tiles = (int *)malloc(sizeof(int) * ntiles);
+    if (hr == NULL || hE == NULL || hF == NULL || hp == NULL || tiles == NULL) {
+        free(cnt); free(start); free(hr); free(hE); free(hF); free(hp); free(tiles);
+        return gerror_(engine_err_malloc, "malloc failed");
+    }
+    {
+        long long off = 0;
+        long long *fill = (long long *)calloc(np + 1, sizeof(long long));
+        for (k = 0; k < np; k++) {
+            start[k] = off;
+            off += 16 * ((cnt[k] + 15) / 16);
+        }
+        for (k = 0; k < n; k++) {
+            const int p = pair[k];
+            const long long d = start[p] + fill[p]++;
+            hr[d] = r[k];
+            hE[d] = E[k];
+            hF[d] = F[k];
+            hp[d] = p;
+        }
+        for (k = 0; k < np; k++)
+            for (long long q = cnt[k]; q < 16 * ((cnt[k] + 15) / 16); q++) {
+                const long long d = start[k] + q, src = start[k] + (q - cnt[k]) % cnt[k];
+                hr[d] = hr[src];
+                hE[d] = hE[src];
+                hF[d] = hF[src];
+                hp[d] = (int)k;
+            }
+        free(fill);
+    }
+    for (k = 0; k < ntiles; k++) tiles[k] = (int)k;
+    free(cnt);
+    free(start);
+    if (mlp_samples(g, m) < 0) {
+        free(hr); free(hE); free(hF); free(hp); free(tiles);
+        return engine_err_gpu;
+    }
+    (void)hipFree(g->d_loss);
+    g->d_loss = NULL;
+    if (hipMalloc(&g->d_loss, sizeof(float) * steps) != hipSuccess ||
+        hipMemcpy(g->d_sr, hr, sizeof(float) * m, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_sp, hp, sizeof(int) * m, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_sE, hE, sizeof(float) * m, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->d_sF, hF, sizeof(float) * m, hipMemcpyHostToDevice) != hipSuccess) {
+        free(hr); free(hE); free(hF); free(hp); free(tiles);
+        return gerror_(engine_err_gpu, "sample upload failed");
+    }
+    free(hr); free(hE); free(hF); free(hp);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(16), dim3(256), 0, g->stream, g->d_loss, (long long)steps, 0.0f);
+    if (batch_tiles > ntiles) batch_tiles = (int)ntiles;
+    pos = (int)ntiles; /* forces a shuffle before the first step */
+    for (step = 0; step < steps; step++) {
+        int nb;
+        double lr, c1, c2;
+        if (pos >= ntiles) {
+            for (k = ntiles - 1; k > 0; k--) {
+                const long long j = (long long)(mlp_rng(&rs) % (unsigned long long)(k + 1));
+                const int t = tiles[k];
+                tiles[k] = tiles[j];
+                tiles[j] = t;
+            }
+            /* the previous step's kernels may still read d_tiles */
+            if (hipStreamSynchronize(g->stream) != hipSuccess ||
+                hipMemcpy(g->d_tiles, tiles, sizeof(int) * ntiles, hipMemcpyHostToDevice) != hipSuccess) {
+                free(tiles);
+                return gerror_(engine_err_gpu, "tile upload failed");
+            }
+            pos = 0;
+        }
+        nb = (int)(ntiles - pos < batch_tiles ? ntiles - pos : batch_tiles);
+        g->mlp_step += 1;
+        lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + cos(3.141592653589793 * step / steps));
+        c1 = 1.0 / (1.0 - pow(b1, (double)g->mlp_step));
+        c2 = 1.0 / (1.0 - pow(b2, (double)g->mlp_step));
+        if (mlp_gradient(g, g->d_tiles + pos, nb, (float)wE, (float)wF, g->d_loss + step) < 0) {
+            free(tiles);
+            return engine_err_gpu;
+        }
+        hipLaunchKernelGGL(k_mlp_adam, dim3((g->mlp_nparams + 255) / 256), dim3(256), 0, g->stream, g->d_theta,
+                           g->d_grad, g->d_adam_m, g->d_adam_v, g->mlp_nparams, (float)lr, b1, b2, (float)c1,
+                           (float)c2, 1e-8f);
+        if (mlp_publish(g) < 0) {
+            free(tiles);
+            return engine_err_gpu;
+        }
+        pos += nb;
+    }
+    free(tiles);
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    HIPCHECK(hipGetLastError());
+    if (loss_hist) HIPCHECK(hipMemcpy(loss_hist, g->d_loss, sizeof(float) * steps, hipMemcpyDeviceToHost));
+    return engine_err_ok;
+}
+
+extern "C" int engine_mlp_eval(struct engine *e, const float *r, const int *pair, long long n, float *E, float *F) {
+    struct gpu_state *g = mlp_state(e, 1);
+    float *dE = NULL, *dF = NULL;
+    int grid;
+    if (g == NULL) return engine_err;
+    if (r == NULL || pair == NULL || E == NULL || F == NULL) return gerror_(engine_err_null, "NULL");
+    if (n < 1) return gerror_(engine_err_range, "n must be positive");
+    if (mlp_check(g, r, pair, n) < 0) return engine_err_range;
+    if (mlp_samples(g, n) < 0) return engine_err_gpu;
+    HIPCHECK(hipMemcpy(g->d_sr, r, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(g->d_sp, pair, sizeof(int) * n, hipMemcpyHostToDevice));
+    dE = g->d_sE;
+    dF = g->d_sF;
+    grid = (int)((n + 15) / 16 < 65536 ? (n + 15) / 16 : 65536);
+    hipLaunchKernelGGL(k_mlp_eval, dim3(grid), dim3(64), 0, g->stream, gpu_mlp_net(g), g->d_sr, g->d_sp, n, dE, dF);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    HIPCHECK(hipMemcpy(E, dE, sizeof(float) * n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(F, dF, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return engine_err_ok;
+}

@@ -26,6 +26,7 @@ engine_flag_gpu = 4
 engine_flag_cuda = 4
 engine_flag_nullpart = 32768
 engine_flag_projwindow = 65536
+engine_flag_mlp = 131072
 
 space_periodic_full = 7
 
@@ -37,6 +38,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _up = C.POINTER(C.c_uint)
 _llp = C.POINTER(C.c_longlong)
+_fp = C.POINTER(C.c_float)
 
 _lib = None
 
@@ -60,6 +62,7 @@ def library():
         "engine_free": (None, [C.c_void_p]),
         "engine_sizeof": (C.c_ulong, []),
         "engine_setdt": (C.c_int, [C.c_void_p, C.c_double]),
+        "engine_setflags": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint]),
         "engine_init": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_uint, C.c_int, C.c_uint]),
         "engine_addtype": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_char_p, C.c_char_p]),
         "engine_addpot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
@@ -74,6 +77,7 @@ def library():
         "engine_finalize": (C.c_int, [C.c_void_p]),
         "engine_getforces": (C.c_int, [C.c_void_p, _dp, C.c_int]),
         "engine_getpositions": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+        "engine_getvelocities": (C.c_int, [C.c_void_p, _dp, C.c_int]),
         "engine_getenergies": (C.c_int, [C.c_void_p, _dp, _llp]),
         "engine_getsortlist": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up]),
         "engine_getcells": (C.c_int, [C.c_void_p, _ip, _ip]),
@@ -86,6 +90,14 @@ def library():
         "engine_gpu_finalize": (C.c_int, [C.c_void_p]),
         "engine_gpu_time": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
         "engine_gpu_results": (C.c_int, [C.c_void_p, _dp, _llp]),
+        "engine_mlp_pairs": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int]),
+        "engine_mlp_init": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_ulonglong]),
+        "engine_mlp_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
+        "engine_mlp_train": (C.c_int, [C.c_void_p, _fp, _ip, _fp, _fp, C.c_longlong, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_ulonglong, _fp]),
+        "engine_mlp_grad": (C.c_int, [C.c_void_p, _fp, _ip, _fp, _fp, C.c_longlong, C.c_double, C.c_double, _fp,
+                                      _dp]),
+        "engine_mlp_eval": (C.c_int, [C.c_void_p, _fp, _ip, C.c_longlong, _fp, _fp]),
         "potential_create_LJ126": (C.c_void_p, [C.c_double] * 5),
         "potential_create_LJ126_switch": (C.c_void_p, [C.c_double] * 6),
         "potential_create_LJ126_Coulomb": (C.c_void_p, [C.c_double] * 6),
@@ -97,6 +109,9 @@ def library():
         "potential_getdata": (C.c_int, [C.c_void_p, _ip, _dp, _dp, _dp, C.c_int]),
         "potential_eval": (None, [C.c_void_p, C.c_double, _dp, _dp]),
         "potential_free": (None, [C.c_void_p]),
+        "potential_eval_r": (C.c_int, [C.c_void_p, _dp, C.c_longlong, _dp, _dp]),
+        "md_sample_pairs": (C.c_longlong, [_dp, _ip, C.c_int, _dp, C.c_double, C.c_longlong, C.c_uint64, _dp, _ip,
+                                           _ip]),
         "md_gen_fcc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp]),
         "md_gen_maxwell": (C.c_int, [C.c_int, _dp, C.c_double, C.c_uint64, _dp]),
         "md_gen_water": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -159,6 +174,14 @@ class Potential:
         f = C.c_double(0.0)
         library().potential_eval(self.handle, float(r2), C.byref(e), C.byref(f))
         return e.value, f.value
+
+    def eval_r(self, r):
+        """E and dE/dr at the distances r (FP64 table evaluation in the library)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        e = np.zeros_like(r)
+        d = np.zeros_like(r)
+        check(library().potential_eval_r(self.handle, ptr(r), len(r), ptr(e), ptr(d)), "potential_eval_r")
+        return e, d
 
 
 def LJ126(a, b, A, B, tol):
@@ -247,6 +270,9 @@ class Engine:
     def step(self):
         check(self.lib.engine_step(self.e), "engine_step")
 
+    def setflags(self, set=0, clear=0):
+        check(self.lib.engine_setflags(self.e, set, clear), "engine_setflags")
+
     def setdt(self, dt):
         check(self.lib.engine_setdt(self.e, dt), "engine_setdt")
 
@@ -270,6 +296,11 @@ class Engine:
         x = np.zeros(3 * self.N)
         check(self.lib.engine_getpositions(self.e, ptr(x), self.N), "engine_getpositions")
         return x.reshape(-1, 3)
+
+    def velocities(self):
+        v = np.zeros(3 * self.N)
+        check(self.lib.engine_getvelocities(self.e, ptr(v), self.N), "engine_getvelocities")
+        return v.reshape(-1, 3)
 
     def energies(self):
         en = np.zeros(4)
@@ -309,6 +340,66 @@ class Engine:
         out = np.zeros(max(count, 1), dtype=np.uint32)
         check(self.lib.engine_getsortlist(self.e, cid, sid, ptr(out, C.c_uint)), "engine_getsortlist")
         return out[:count]
+
+    # ------------------------------------------------------------ MLP surrogate
+
+    def mlp_pairs(self):
+        """Species pairs (ti, tj) in pair-id order."""
+        n = check(self.lib.engine_mlp_pairs(self.e, None, None, 0), "engine_mlp_pairs")
+        ti = np.zeros(max(n, 1), np.int32)
+        tj = np.zeros(max(n, 1), np.int32)
+        check(self.lib.engine_mlp_pairs(self.e, ptr(ti, C.c_int), ptr(tj, C.c_int), n), "engine_mlp_pairs")
+        return ti[:n], tj[:n]
+
+    def mlp_init(self, r_lo, r_hi, escale, fscale, seed=1):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (r_lo, r_hi, escale, fscale)]
+        return check(self.lib.engine_mlp_init(self.e, *[ptr(v) for v in a], seed), "engine_mlp_init")
+
+    def mlp_params(self, theta=None):
+        """Get (theta None) or set the fp32 parameter vector."""
+        n = check(self.lib.engine_mlp_params(self.e, None, 1), "engine_mlp_params")
+        if theta is None:
+            out = np.zeros(n, np.float32)
+            check(self.lib.engine_mlp_params(self.e, ptr(out, C.c_float), 1), "engine_mlp_params")
+            return out
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        assert t.size == n
+        check(self.lib.engine_mlp_params(self.e, ptr(t, C.c_float), 0), "engine_mlp_params")
+        return t
+
+    @staticmethod
+    def _samples(r, pair, E=None, F=None):
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        pair = np.ascontiguousarray(pair, dtype=np.int32)
+        E = None if E is None else np.ascontiguousarray(E, dtype=np.float32)
+        F = None if F is None else np.ascontiguousarray(F, dtype=np.float32)
+        return r, pair, E, F
+
+    def mlp_train(self, r, pair, E, F, steps, batch_tiles=512, lr0=3e-3, lr1=3e-5, wE=1.0, wF=1.0, seed=1):
+        r, pair, E, F = self._samples(r, pair, E, F)
+        hist = np.zeros(steps, np.float32)
+        check(self.lib.engine_mlp_train(self.e, ptr(r, C.c_float), ptr(pair, C.c_int), ptr(E, C.c_float),
+                                        ptr(F, C.c_float), len(r), steps, batch_tiles, lr0, lr1, wE, wF, seed,
+                                        ptr(hist, C.c_float)), "engine_mlp_train")
+        return hist
+
+    def mlp_grad(self, r, pair, E, F, wE=1.0, wF=1.0):
+        r, pair, E, F = self._samples(r, pair, E, F)
+        n = check(self.lib.engine_mlp_params(self.e, None, 1), "engine_mlp_params")
+        g = np.zeros(n, np.float32)
+        loss = np.zeros(1)
+        check(self.lib.engine_mlp_grad(self.e, ptr(r, C.c_float), ptr(pair, C.c_int), ptr(E, C.c_float),
+                                       ptr(F, C.c_float), len(r), wE, wF, ptr(g, C.c_float), ptr(loss)),
+              "engine_mlp_grad")
+        return g, float(loss[0])
+
+    def mlp_eval(self, r, pair):
+        r, pair, _, _ = self._samples(r, pair)
+        E = np.zeros(len(r), np.float32)
+        F = np.zeros(len(r), np.float32)
+        check(self.lib.engine_mlp_eval(self.e, ptr(r, C.c_float), ptr(pair, C.c_int), len(r), ptr(E, C.c_float),
+                                       ptr(F, C.c_float)), "engine_mlp_eval")
+        return E, F
 
     def close(self):
         if self.e:
@@ -358,6 +449,21 @@ def gen_types(N, ntypes, first, seed):
     t = np.zeros(N, dtype=np.int32)
     check(library().md_gen_types(N, ntypes, first, seed, ptr(t, C.c_int)), "md_gen_types")
     return t
+
+
+def sample_pairs(x, types, box, cutoff, nmax, seed):
+    """In-cutoff pair distances of a periodic configuration: (r, type_i, type_j)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    r = np.zeros(nmax)
+    ti = np.zeros(nmax, np.int32)
+    tj = np.zeros(nmax, np.int32)
+    n = library().md_sample_pairs(ptr(x), ptr(t, C.c_int), len(t), ptr(b), cutoff, nmax, seed, ptr(r),
+                                  ptr(ti, C.c_int), ptr(tj, C.c_int))
+    if n < 0:
+        raise MdcoreError("md_sample_pairs: bad input (box must hold 3 cutoffs per axis)")
+    return r[:n], ti[:n], tj[:n]
 
 
 def gen_uniform(n, lo, hi, seed):

@@ -63,6 +63,13 @@ static double wall_seconds(void) {
 
 unsigned long engine_sizeof(void) { return sizeof(struct engine); }
 
+int engine_setflags(struct engine *e, unsigned int set, unsigned int clear) {
+    if (e == NULL) return eerror_(engine_err_null);
+    if (((set | clear) & engine_flag_gpu) && e->gpu != NULL) return eerror_(engine_err_range);
+    e->flags = (e->flags | set) & ~clear;
+    return engine_err_ok;
+}
+
 int engine_setdt(struct engine *e, double dt) {
     if (e == NULL) return eerror_(engine_err_null);
     if (!(dt >= 0.0)) return eerror_(engine_err_range);
@@ -347,6 +354,17 @@ int engine_getpositions(struct engine *e, double *x, int N) {
     if (e == NULL || x == NULL) return eerror_(engine_err_null);
     for (id = 0; id < N; id++)
         if (space_getpos(&e->s, id, &x[3 * id]) < 0) return eerror_(engine_err_space);
+    return engine_err_ok;
+}
+
+int engine_getvelocities(struct engine *e, double *v, int N) {
+    int id, k;
+    if (e == NULL || v == NULL) return eerror_(engine_err_null);
+    for (id = 0; id < N; id++) {
+        const struct part *p = (id < e->s.size_parts) ? e->s.partlist[id] : NULL;
+        if (p == NULL) return eerror_(engine_err_range);
+        for (k = 0; k < 3; k++) v[3 * id + k] = p->v[k];
+    }
     return engine_err_ok;
 }
 

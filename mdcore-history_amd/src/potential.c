@@ -580,6 +580,20 @@ void potential_eval(const struct potential *p, FPTYPE r2, FPTYPE *e, FPTYPE *f) 
     *f = eff * c[1] / r;
 }
 
+/* Energy and dE/dr at the distances r[0..n) (a vector helper for callers
+   that sample a table, e.g. to build training data). */
+int potential_eval_r(const struct potential *p, const double *r, long long n, double *e, double *dedr) {
+    long long k;
+    FPTYPE ee, ff;
+    if (p == NULL || r == NULL || e == NULL || dedr == NULL) return perror_(potential_err_null);
+    for (k = 0; k < n; k++) {
+        potential_eval(p, r[k] * r[k], &ee, &ff);
+        e[k] = ee;
+        dedr[k] = ff * r[k];
+    }
+    return potential_err_ok;
+}
+
 /* ------------------------------------------------------------ constructors */
 
 /* The reference passes the parameters to potential_init's callbacks through

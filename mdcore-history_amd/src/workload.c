@@ -5,6 +5,8 @@
  *   md_gen_water      3-site water molecules on a lattice, random orientation
  *   md_gen_rsa        random sequential addition with a minimum separation
  *   md_gen_types      uniform random species labels
+ *   md_sample_pairs   in-cutoff pair distances of a configuration (training
+ *                     samples for the MLP surrogate)
  * These are drivers' helpers (the reference builds its systems inside the
  * example programs, e.g. examples/argon/argon.c:160-200); they are not part
  * of the nonbonded path itself.
@@ -208,4 +210,85 @@ int md_gen_uniform(int n, double lo, double hi, uint64_t seed, double *out) {
     if (out == NULL) return -1;
     for (i = 0; i < n; i++) out[i] = lo + (hi - lo) * md_uniform(&s);
     return n;
+}
+
+/* In-cutoff pairs of a periodic configuration: atoms i are visited in a
+   seeded random order and every j != i within the cutoff (minimum image) is
+   emitted as (r, type[i], type[j]) until nmax pairs are written.  The
+   distances therefore follow the configuration's own pair distribution.
+   Returns the number written, or -1 on bad input or allocation failure. */
+long long md_sample_pairs(const double *x, const int *type, int N, const double *box, double cutoff,
+                          long long nmax, uint64_t seed, double *r, int *ti, int *tj) {
+    int nc[3], k, ncell;
+    int *head, *next, *order;
+    long long w = 0;
+    uint64_t s = seed;
+    if (x == NULL || type == NULL || box == NULL || r == NULL || ti == NULL || tj == NULL || N < 2 || !(cutoff > 0))
+        return -1;
+    for (k = 0; k < 3; k++) {
+        nc[k] = (int)floor(box[k] / cutoff);
+        if (nc[k] < 3) return -1;
+    }
+    ncell = nc[0] * nc[1] * nc[2];
+    head = (int *)malloc(sizeof(int) * ncell);
+    next = (int *)malloc(sizeof(int) * N);
+    order = (int *)malloc(sizeof(int) * N);
+    if (head == NULL || next == NULL || order == NULL) {
+        free(head);
+        free(next);
+        free(order);
+        return -1;
+    }
+    for (k = 0; k < ncell; k++) head[k] = -1;
+    for (int i = 0; i < N; i++) {
+        int c[3];
+        for (k = 0; k < 3; k++) {
+            double u = x[3 * i + k] - floor(x[3 * i + k] / box[k]) * box[k];
+            c[k] = (int)(u / box[k] * nc[k]);
+            if (c[k] >= nc[k]) c[k] = nc[k] - 1;
+        }
+        const int cid = (c[0] * nc[1] + c[1]) * nc[2] + c[2];
+        next[i] = head[cid];
+        head[cid] = i;
+        order[i] = i;
+    }
+    for (int i = N - 1; i > 0; i--) {
+        const int j = (int)(sm_next(&s) % (uint64_t)(i + 1));
+        const int t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+    }
+    for (int q = 0; q < N && w < nmax; q++) {
+        const int i = order[q];
+        int c[3];
+        for (k = 0; k < 3; k++) {
+            double u = x[3 * i + k] - floor(x[3 * i + k] / box[k]) * box[k];
+            c[k] = (int)(u / box[k] * nc[k]);
+            if (c[k] >= nc[k]) c[k] = nc[k] - 1;
+        }
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++) {
+                    const int cx = (c[0] + dx + nc[0]) % nc[0], cy = (c[1] + dy + nc[1]) % nc[1],
+                              cz = (c[2] + dz + nc[2]) % nc[2];
+                    for (int j = head[(cx * nc[1] + cy) * nc[2] + cz]; j >= 0 && w < nmax; j = next[j]) {
+                        double d2 = 0.0;
+                        if (j == i) continue;
+                        for (k = 0; k < 3; k++) {
+                            double d = x[3 * i + k] - x[3 * j + k];
+                            d -= box[k] * nearbyint(d / box[k]);
+                            d2 += d * d;
+                        }
+                        if (d2 > cutoff * cutoff) continue;
+                        r[w] = sqrt(d2);
+                        ti[w] = type[i];
+                        tj[w] = type[j];
+                        w++;
+                    }
+                }
+    }
+    free(head);
+    free(next);
+    free(order);
+    return w;
 }

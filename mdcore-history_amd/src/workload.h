@@ -16,6 +16,8 @@ int md_gen_water(int nmol, const double *box, int nlx, int nly, int nlz, double 
 int md_gen_rsa(int N, const double *box, double rmin, uint64_t seed, long long max_trials, double *x);
 int md_gen_types(int N, int ntypes, int first, uint64_t seed, int *type);
 int md_gen_uniform(int n, double lo, double hi, uint64_t seed, double *out);
+long long md_sample_pairs(const double *x, const int *type, int N, const double *box, double cutoff,
+                          long long nmax, uint64_t seed, double *r, int *ti, int *tj);
 
 #ifdef __cplusplus
 }

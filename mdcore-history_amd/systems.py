@@ -101,17 +101,32 @@ def mixture(N=1048576, density=100.0, seed=7, rmin=0.15, cutoff=1.2, cell=0.6, n
     counts = np.bincount(types - 1, minlength=nspecies)
     q = q - np.dot(counts, q) / N  # count-weighted shift: sum_i q_i = 0
     x = md.gen_rsa(N, [box] * 3, rmin, seed + 20)
+    pots = mixture_pots(sig, eps, q, cutoff, a, eps_rf, tol)
+    return System("mixture", np.zeros(3), np.full(3, box), np.full(3, cell), cutoff, x, types,
+                  np.concatenate([[0.0], mass]), pots, None, None,
+                  {"box": box, "sigma": sig, "eps": eps, "q": q, "density": density, "a": a, "eps_rf": eps_rf})
+
+
+def mixture_pots(sig, eps, q, cutoff, a, eps_rf, tol):
+    """LJ 12-6 (Lorentz-Berthelot) + reaction-field tables of every species pair."""
     pots = []
-    for s in range(nspecies):
-        for t in range(s, nspecies):
+    n = len(sig)
+    for s in range(n):
+        for t in range(s, n):
             sg = 0.5 * (sig[s] + sig[t])
             ep = math.sqrt(eps[s] * eps[t])
             A = 4 * ep * sg ** 12
             B = 4 * ep * sg ** 6
             pots.append((s + 1, t + 1, md.LJ126_RF(a, cutoff, A, B, q[s] * q[t], KE, eps_rf, tol)))
-    return System("mixture", np.zeros(3), np.full(3, box), np.full(3, cell), cutoff, x, types,
-                  np.concatenate([[0.0], mass]), pots, None, None,
-                  {"box": box, "sigma": sig, "eps": eps, "q": q, "density": density})
+    return pots
+
+
+def with_tolerance(sys, tol):
+    """The same mixture with its tables rebuilt at another tolerance."""
+    m = sys.meta
+    pots = mixture_pots(m["sigma"], m["eps"], m["q"], sys.cutoff, m["a"], m["eps_rf"], tol)
+    return System(sys.name, sys.origin, sys.dim, sys.L, sys.cutoff, sys.x, sys.types, sys.masses, pots, sys.excl,
+                  sys.v, dict(m, tol=tol))
 
 
 def build_engine(sys, flags=md.engine_flag_gpu, start=True):
