@@ -281,9 +281,11 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
         const int qv = q < T ? q : T - 1;
         const float r = sqrtf(dn_r2[qv]);
         const int p = dn_p[qv];
-        float z1[16], z2[16], h2[16], y0, y1;
-        mlp_bf16x8 h1b[2];
-        mlp_forward16(N, r, p, lane, z1, h1b, z2, h2, y0, y1);
+        float y0, y1;
+        /* keep the network operands' loads inside the loop: hoisting them
+           out keeps ~100 registers live through the whole pair kernel */
+        asm volatile("" ::: "memory");
+        mlp_infer16(N, r, p, lane, y0, y1);
         if (lane < 16 && q < T) {
             float e, f;
             mlp_energy_force(N.sc[p], r, y0, y1, e, f);
@@ -368,9 +370,13 @@ __device__ __forceinline__ void nonbond_body(
         for (int ch = 0; ch < nchunk; ch++) {
             const int i0 = (int)(((long long)ch * nc) / nchunk);
             const int nsub = (int)(((long long)(ch + 1) * nc) / nchunk) - i0;
-            const int NP = 64 / nsub;
+            /* Lane = (atom il, partition part): every lane works.  Atom il has
+               NPi = (63 - il) / nsub + 1 partitions (lanes il, il + nsub, ...),
+               each walking every NPi-th record of the atom's window. */
             const int il = lane % nsub, part = lane / nsub;
-            const bool active = part < NP;
+            const int NPi = (63 - il) / nsub + 1;
+            const int NPmax = 63 / nsub + 1;
+            const bool active = part < NPi;
             const int myidx = i0 + il; /* storage index in the owner cell */
             const int a = offc + myidx;
             const float4 xi = xf[a];
@@ -499,7 +505,7 @@ __device__ __forceinline__ void nonbond_body(
                                 break;
                             }
                         } else if (t == myidx) {
-                            t += NP;
+                            t += NPi;
                             continue;
                         }
                         const float dx = (xi.x - pj.x) - sf0, dy = (xi.y - pj.y) - sf1, dz = (xi.z - pj.z) - sf2;
@@ -526,7 +532,7 @@ __device__ __forceinline__ void nonbond_body(
                             hitl[h * 64 + lane] = (unsigned short)t;
                             h++;
                         }
-                        t += NP;
+                        t += NPi;
                     }
 
                     if constexpr (!MLP) {
@@ -762,11 +768,11 @@ __device__ __forceinline__ void nonbond_body(
             }
 
             /* Fold the partitions of each atom into partition 0, in order. */
-            for (int p = 1; p < NP; p++) {
+            for (int p = 1; p < NPmax; p++) {
                 const int src = (lane + p * nsub) & 63;
                 const float ohx = __shfl(hx, src), ohy = __shfl(hy, src), ohz = __shfl(hz, src);
                 const float olx = __shfl(lx, src), oly = __shfl(ly, src), olz = __shfl(lz, src);
-                if (part == 0) {
+                if (part == 0 && p < NPi) {
                     acc2(hx, lx, ohx);
                     acc2(hy, ly, ohy);
                     acc2(hz, lz, ohz);

@@ -244,9 +244,8 @@ __global__ __launch_bounds__(64) void k_mlp_eval(struct mlp_net N, const float *
         const long long sv = s < n ? s : n - 1;
         const int p = pair[sv];
         const float rr = r[sv];
-        float z1[16], z2[16], h2[16], y0, y1;
-        mlp_bf16x8 h1b[2];
-        mlp_forward16(N, rr, p, lane, z1, h1b, z2, h2, y0, y1);
+        float y0, y1;
+        mlp_infer16(N, rr, p, lane, y0, y1);
         if (lane < 16 && s < n) {
             float e, f;
             mlp_energy_force(N.sc[p], rr, y0, y1, e, f);

@@ -84,45 +84,57 @@ __device__ __forceinline__ void mlp_inputs(const float4 sc, float r, float &rn, 
 
 /* Forward pass of a 16-pair tile.  Every lane passes the (r, p) of pair
    l % 16 (lanes l, l+16, l+32, l+48 pass the same pair).  Results: y0, y1
-   of that pair in every lane; z1/z2/h1/h2 of this lane's 16 units kept for
-   the backward pass (the caller may ignore them). */
-__device__ __forceinline__ void mlp_forward16(const mlp_net &N, float r, int p, int lane, float z1[16],
-                                              mlp_bf16x8 h1b[2], float z2[16], float h2[16], float &y0, float &y1) {
+   of that pair in every lane.  With KEEP, the lane's z1/z2/h2 and the bf16
+   h1 operands are kept for the backward pass; without it (inference) each
+   16-unit block goes layer 2 -> SiLU -> heads as soon as its MFMA is done,
+   so few registers stay live.  Both variants do the same arithmetic in the
+   same order, so they give the same bits. */
+template <bool KEEP>
+__device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, float r, int p, int lane, float *z1,
+                                                mlp_bf16x8 *h1b, float *z2, float *h2, float &y0, float &y1) {
     const float4 sc = N.sc[p];
     float rn, irn;
     mlp_inputs(sc, r, rn, irn);
     const float *b1e = N.b1e + (size_t)p * MLP_H;
+    mlp_bf16x8 hb[2];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int u = mlp_unit(j, lane);
         const float2 w = *(const float2 *)(N.w1 + 2 * u);
-        z1[j] = w.x * rn + w.y * irn + b1e[u];
+        const float z = w.x * rn + w.y * irn + b1e[u];
+        if (KEEP) z1[j] = z;
+        hb[j >> 3][j & 7] = (__bf16)mlp_silu(z);
     }
-#pragma unroll
-    for (int kb = 0; kb < 2; kb++)
-#pragma unroll
-        for (int e = 0; e < 8; e++) h1b[kb][e] = (__bf16)mlp_silu(z1[8 * kb + e]);
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) {
-        mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2f[(mb * 2 + 0) * 64 + lane], h1b[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2f[(mb * 2 + 1) * 64 + lane], h1b[1], acc, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int j = 4 * mb + i;
-            z2[j] = acc[i] + N.b2[mlp_unit(j, lane)];
-            h2[j] = mlp_silu(z2[j]);
-        }
+    if (KEEP) {
+        h1b[0] = hb[0];
+        h1b[1] = hb[1];
     }
     const float *w3 = N.w3 + (size_t)p * 2 * MLP_H;
     float s0 = 0.f, s1 = 0.f;
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
+        mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2f[(mb * 2 + 0) * 64 + lane], hb[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2f[(mb * 2 + 1) * 64 + lane], hb[1], acc, 0, 0, 0);
         const int u0 = 16 * mb + 4 * (lane >> 4);
+        const float4 bb = *(const float4 *)(N.b2 + u0);
+        float hh[4];
+        hh[0] = mlp_silu(acc[0] + bb.x);
+        hh[1] = mlp_silu(acc[1] + bb.y);
+        hh[2] = mlp_silu(acc[2] + bb.z);
+        hh[3] = mlp_silu(acc[3] + bb.w);
+        if (KEEP) {
+            z2[4 * mb] = acc[0] + bb.x;
+            z2[4 * mb + 1] = acc[1] + bb.y;
+            z2[4 * mb + 2] = acc[2] + bb.z;
+            z2[4 * mb + 3] = acc[3] + bb.w;
+#pragma unroll
+            for (int i = 0; i < 4; i++) h2[4 * mb + i] = hh[i];
+        }
         const float4 a = *(const float4 *)(w3 + u0);
         const float4 b = *(const float4 *)(w3 + MLP_H + u0);
-        s0 += h2[4 * mb] * a.x + h2[4 * mb + 1] * a.y + h2[4 * mb + 2] * a.z + h2[4 * mb + 3] * a.w;
-        s1 += h2[4 * mb] * b.x + h2[4 * mb + 1] * b.y + h2[4 * mb + 2] * b.z + h2[4 * mb + 3] * b.w;
+        s0 += hh[0] * a.x + hh[1] * a.y + hh[2] * a.z + hh[3] * a.w;
+        s1 += hh[0] * b.x + hh[1] * b.y + hh[2] * b.z + hh[3] * b.w;
     }
     s0 += __shfl_xor(s0, 16);
     s1 += __shfl_xor(s1, 16);
@@ -130,6 +142,17 @@ __device__ __forceinline__ void mlp_forward16(const mlp_net &N, float r, int p, 
     s1 += __shfl_xor(s1, 32);
     y0 = s0 + N.b3[2 * p];
     y1 = s1 + N.b3[2 * p + 1];
+}
+
+/* Training forward: keeps the activations of this lane's 16 units. */
+__device__ __forceinline__ void mlp_forward16(const mlp_net &N, float r, int p, int lane, float z1[16],
+                                              mlp_bf16x8 h1b[2], float z2[16], float h2[16], float &y0, float &y1) {
+    mlp_forward16_t<true>(N, r, p, lane, z1, h1b, z2, h2, y0, y1);
+}
+
+/* Inference forward. */
+__device__ __forceinline__ void mlp_infer16(const mlp_net &N, float r, int p, int lane, float &y0, float &y1) {
+    mlp_forward16_t<false>(N, r, p, lane, nullptr, nullptr, nullptr, nullptr, y0, y1);
 }
 
 __device__ __forceinline__ float mlp_sinh(float y) {
