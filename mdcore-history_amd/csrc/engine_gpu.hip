@@ -261,6 +261,15 @@ struct nb_out {
     unsigned long long *cell_cnt;
 };
 
+/* True if p holds on any lane (scalar test of the wave's ballot mask). */
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+/* Compile-time tag of a visit kind (self / owner is i / owner is j). */
+template <int V>
+struct visit_kind {
+    static constexpr int value = V;
+};
+
 /* Exclusive prefix sum of v over the wave. */
 __device__ __forceinline__ int wave_excl_scan(int v, int lane) {
     int x = v;
@@ -416,7 +425,8 @@ __device__ __forceinline__ void nonbond_body(
                 /* Stage the neighbour, in the order its window is walked:
                    ascending keys when the owner is the task's i cell,
                    descending otherwise, storage order for the self task.
-                   Record = (x, y, z, key | index << 16 | type << 26). */
+                   Record = (x, y, z, key' | index << 16 | type << 26) with
+                   key' = key, or 0xffff - key when the owner is the j cell. */
                 wave_lds_sync();
                 if (lane < nj) raw[lane] = pfx;
                 wave_lds_sync();
@@ -428,8 +438,9 @@ __device__ __forceinline__ void nonbond_body(
                     else {
                         const int idx = (int)(pfw >> 16) < nj ? (int)(pfw >> 16) : nj - 1;
                         const float4 p = idx < 64 ? raw[idx] : xf[offj + idx];
+                        const unsigned int kk = orient ? 0xffffu - (pfw & 0xffffu) : (pfw & 0xffffu);
                         stage[lane] = make_float4(p.x, p.y, p.z,
-                                                  __uint_as_float((pfw & 0xffffu) | ((unsigned int)idx << 16) |
+                                                  __uint_as_float(kk | ((unsigned int)idx << 16) |
                                                                   ((unsigned int)__float_as_int(p.w) << 26)));
                     }
                 }
@@ -444,8 +455,9 @@ __device__ __forceinline__ void nonbond_body(
                             sortlist[13 * (size_t)offj + (size_t)sid * nj + (orient ? t : nj - 1 - t)];
                         const int idx = (int)(w >> 16) < nj ? (int)(w >> 16) : nj - 1;
                         const float4 p = xf[offj + idx];
+                        const unsigned int kk = orient ? 0xffffu - (w & 0xffffu) : (w & 0xffffu);
                         stage[t] = make_float4(p.x, p.y, p.z,
-                                               __uint_as_float((w & 0xffffu) | ((unsigned int)idx << 16) |
+                                               __uint_as_float(kk | ((unsigned int)idx << 16) |
                                                                ((unsigned int)__float_as_int(p.w) << 26)));
                     }
                 }
@@ -475,170 +487,186 @@ __device__ __forceinline__ void nonbond_body(
                 bool live_ref = true, run = true;
                 if (!self) {
                     dns = P.proj ? en.w : en.z;
-                    const int kstage0 = (int)(__float_as_uint(stage[0].w) & 0xffffu);
+                    const int k0 = (int)(__float_as_uint(stage[0].w) & 0xffffu);
+                    const int kstage0 = orient ? 0xffff - k0 : k0; /* staged keys are complemented when orient */
                     const int s_imax = orient ? kstage0 : kfl[sid];
                     const int s_jmin = orient ? kfl[13 + sid] : kstage0;
                     live_ref = (s_jmin + en.z - s_imax) >= 0;
                     run = P.proj || live_ref;
                     myk = own_key[sid * 64 + il];
                 }
+                /* Window (runner_dopair.c:455-478): record t is admitted while
+                   key_j + dnshift - key_i < dmaxdist (owner is i) or
+                   key_i + dnshift - key_j < dmaxdist (owner is j).  With the
+                   staged key k' = key (owner is i) or 0xffff - key (owner is
+                   j), both read k' < kmax, and k' ascends along the staged
+                   order, so a lane's window ends at its first failing record. */
+                const int kmax = orient ? P.dmaxdist - (myk + dns - 0xffff) : P.dmaxdist - (dns - myk);
 
                 float lfx = 0.f, lfy = 0.f, lfz = 0.f, le = 0.f, lv = 0.f;
                 unsigned int lnp = 0, lnref = 0;
-                int t = part;
-                bool more = active && run;
-                while (__any(more)) {
-                    /* Filter: walk this lane's part of the window, keep hits. */
-                    int h = 0;
-                    while (more && h < HCAP) {
-                        if (t >= nj) {
-                            more = false;
-                            break;
-                        }
-                        const float4 pj = stage[t];
-                        const unsigned int wj = __float_as_uint(pj.w);
-                        if (!self) {
-                            const int kj = (int)rec_key(wj);
-                            const int gap = orient ? (myk + dns - kj) : (kj + dns - myk);
-                            if (gap >= P.dmaxdist) {
-                                more = false;
-                                break;
-                            }
-                        } else if (t == myidx) {
-                            t += NPi;
-                            continue;
-                        }
-                        const float dx = (xi.x - pj.x) - sf0, dy = (xi.y - pj.y) - sf1, dz = (xi.z - pj.z) - sf2;
-                        const float r2 = dx * dx + dy * dy + dz * dz;
-                        bool hit = r2 <= P.rc2f;
-                        if (fabsf(r2 - P.rc2f) <= P.band) {
-                            /* Decide in FP64 exactly as the reference does, in
-                               the task's own orientation. */
-                            const double *xa = &x64[3 * (size_t)a];
-                            const double *xb = &x64[3 * (size_t)(offj + rec_idx(wj))];
-                            const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cj];
-                            double r2d;
-                            if (self || orient == 0)
-                                r2d = r2_task64(xa, xb, min_image(o_nb[0] - o_own[0], P.dim[0]),
-                                                min_image(o_nb[1] - o_own[1], P.dim[1]),
-                                                min_image(o_nb[2] - o_own[2], P.dim[2]));
-                            else
-                                r2d = r2_task64(xb, xa, min_image(o_own[0] - o_nb[0], P.dim[0]),
-                                                min_image(o_own[1] - o_nb[1], P.dim[1]),
-                                                min_image(o_own[2] - o_nb[2], P.dim[2]));
-                            hit = r2d <= P.cutoff2;
-                        }
-                        if (hit) {
-                            hitl[h * 64 + lane] = (unsigned short)t;
-                            h++;
-                        }
-                        t += NPi;
-                    }
 
-                    if constexpr (!MLP) {
-                        /* Evaluate the kept hits, two per lane per round. */
-                        for (int s = 0; __any(s < h); s += 2) {
-                            const bool va = s < h, vb = s + 1 < h;
-                            const int ta = va ? hitl[s * 64 + lane] : 0;
-                            const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
-                            const float4 pa = stage[ta], pb = stage[tb];
-                            const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
-                            const float4 ha = ptab[ti * mt + rec_type(wa)], hb = ptab[ti * mt + rec_type(wb)];
-                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
-                            const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1, dzb = (xi.z - pb.z) - sf2;
-                            const float r2a = dxa * dxa + dya * dya + dza * dza;
-                            const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                            const bool ua = va && __float_as_int(ha.w) != -1;
-                            const bool ub = vb && __float_as_int(hb.w) != -1;
-                            float ea = 0.f, fa = 0.f, eb = 0.f, fb = 0.f;
-                            if (ua) pot_eval32(ha, coeffs4, r2a, ea, fa);
-                            if (ub) pot_eval32(hb, coeffs4, r2b, eb, fb);
-                            if (ua) {
+                /* One visit, specialised by kind: 0 self task, 1 owner is the
+                   task's i cell (energies, virial and counts are taken here),
+                   2 owner is the j cell (forces only). */
+                auto visit = [&](auto kind) {
+                    constexpr int VK = decltype(kind)::value;
+                    int t = part;
+                    bool alive = active && run;
+                    while (wave_any(alive)) {
+                        /* Filter: up to HCAP records per lane, predicated; the
+                           wave leaves as soon as no lane is inside its window. */
+                        int h = 0;
+                        for (int it = 0; it < HCAP; it++) {
+                            const float4 pj = stage[t < nj ? t : nj - 1];
+                            const unsigned int wj = __float_as_uint(pj.w);
+                            alive = alive && t < nj;
+                            if (VK != 0) alive = alive && (int)rec_key(wj) < kmax;
+                            const bool cand = VK == 0 ? (alive && t != myidx) : alive;
+                            const float dx = (xi.x - pj.x) - sf0, dy = (xi.y - pj.y) - sf1, dz = (xi.z - pj.z) - sf2;
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            bool hit = cand && r2 <= P.rc2f;
+                            const bool near = cand && fabsf(r2 - P.rc2f) <= P.band;
+                            if (wave_any(near)) {
+                                if (near) {
+                                    /* Decide in FP64 exactly as the reference does, in
+                                       the task's own orientation. */
+                                    const double *xa = &x64[3 * (size_t)a];
+                                    const double *xb = &x64[3 * (size_t)(offj + rec_idx(wj))];
+                                    const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cj];
+                                    double r2d;
+                                    if (VK != 2)
+                                        r2d = r2_task64(xa, xb, min_image(o_nb[0] - o_own[0], P.dim[0]),
+                                                        min_image(o_nb[1] - o_own[1], P.dim[1]),
+                                                        min_image(o_nb[2] - o_own[2], P.dim[2]));
+                                    else
+                                        r2d = r2_task64(xb, xa, min_image(o_own[0] - o_nb[0], P.dim[0]),
+                                                        min_image(o_own[1] - o_nb[1], P.dim[1]),
+                                                        min_image(o_own[2] - o_nb[2], P.dim[2]));
+                                    hit = r2d <= P.cutoff2;
+                                }
+                            }
+                            hitl[h * 64 + lane] = (unsigned short)t;
+                            h += hit ? 1 : 0;
+                            t += NPi;
+                            if (!wave_any(alive)) break;
+                        }
+
+                        if constexpr (!MLP) {
+                            /* Evaluate the kept hits, two per lane per round. */
+                            for (int s = 0; wave_any(s < h); s += 2) {
+                                const bool va = s < h, vb = s + 1 < h;
+                                const int ta = va ? hitl[s * 64 + lane] : 0;
+                                const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
+                                const float4 pa = stage[ta], pb = stage[tb];
+                                const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
+                                const float4 ha = ptab[ti * mt + rec_type(wa)], hb = ptab[ti * mt + rec_type(wb)];
+                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
+                                            dza = (xi.z - pa.z) - sf2;
+                                const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1,
+                                            dzb = (xi.z - pb.z) - sf2;
+                                const float r2a = dxa * dxa + dya * dya + dza * dza;
+                                const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+                                const bool ua = va && __float_as_int(ha.w) != -1;
+                                const bool ub = vb && __float_as_int(hb.w) != -1;
+                                float ea = 0.f, fa = 0.f, eb = 0.f, fb = 0.f;
+                                if (ua) pot_eval32(ha, coeffs4, r2a, ea, fa);
+                                if (ub) pot_eval32(hb, coeffs4, r2b, eb, fb);
+                                lfx -= fa * dxa + fb * dxb;
+                                lfy -= fa * dya + fb * dyb;
+                                lfz -= fa * dza + fb * dzb;
+                                if (VK != 2) {
+                                    const bool ca = ua && (VK == 1 || myidx > rec_idx(wa));
+                                    const bool cb = ub && (VK == 1 || myidx > rec_idx(wb));
+                                    if (ca) {
+                                        le += ea;
+                                        lv -= fa * r2a;
+                                        lnp++;
+                                        lnref += (VK == 0 || !P.proj ||
+                                                  (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                    }
+                                    if (cb) {
+                                        le += eb;
+                                        lv -= fb * r2b;
+                                        lnp++;
+                                        lnref += (VK == 0 || !P.proj ||
+                                                  (live_ref && ((int)rec_key(wb) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                    }
+                                }
+                            }
+                        } else {
+                            /* Below the pair's r_lo: table, by this lane.  Otherwise
+                               keep the hit for the dense list. */
+                            int nd = 0;
+                            for (int s = 0; s < h; s++) {
+                                const int ta = hitl[s * 64 + lane];
+                                const float4 pa = stage[ta];
+                                const unsigned int wa = __float_as_uint(pa.w);
+                                const float4 ha = ptab[ti * mt + rec_type(wa)];
+                                if (__float_as_int(ha.w) == -1) continue;
+                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
+                                            dza = (xi.z - pa.z) - sf2;
+                                const float r2a = dxa * dxa + dya * dya + dza * dza;
+                                const float rlo = N.sc[ptype[ti * mt + rec_type(wa)]].x;
+                                if (r2a >= rlo * rlo) {
+                                    hitl[nd * 64 + lane] = (unsigned short)ta;
+                                    nd++;
+                                    continue;
+                                }
+                                float ea, fa;
+                                pot_eval32(ha, coeffs4, r2a, ea, fa);
                                 lfx -= fa * dxa;
                                 lfy -= fa * dya;
                                 lfz -= fa * dza;
+                                if (VK != 2 && (VK == 1 || myidx > rec_idx(wa))) {
+                                    le += ea;
+                                    lv -= fa * r2a;
+                                    lnp++;
+                                    lnref += (VK == 0 || !P.proj ||
+                                              (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                }
                             }
-                            if (ub) {
-                                lfx -= fb * dxb;
-                                lfy -= fb * dyb;
-                                lfz -= fb * dzb;
+                            /* Dense list: this lane's hits at [off, off + nd). */
+                            const int off = wave_excl_scan(nd, lane);
+                            const int T = __shfl(off + nd, 63);
+                            for (int s = 0; s < nd; s++) {
+                                const float4 pa = stage[hitl[s * 64 + lane]];
+                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
+                                            dza = (xi.z - pa.z) - sf2;
+                                dn_r2[off + s] = dxa * dxa + dya * dya + dza * dza;
+                                dn_p[off + s] = ptype[ti * mt + rec_type(__float_as_uint(pa.w))];
                             }
-                            const bool ca = ua && (self ? (myidx > rec_idx(wa)) : (orient == 0));
-                            const bool cb = ub && (self ? (myidx > rec_idx(wb)) : (orient == 0));
-                            if (ca) {
-                                le += ea;
-                                lv -= fa * r2a;
-                                lnp++;
-                                lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                            wave_lds_sync();
+                            mlp_dense_eval(N, dn_r2, dn_p, dn_f, dn_e, T, lane);
+                            wave_lds_sync();
+                            for (int s = 0; s < nd; s++) {
+                                const float4 pa = stage[hitl[s * 64 + lane]];
+                                const unsigned int wa = __float_as_uint(pa.w);
+                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
+                                            dza = (xi.z - pa.z) - sf2;
+                                const float r2a = dxa * dxa + dya * dya + dza * dza;
+                                const float fa = dn_f[off + s], ea = dn_e[off + s];
+                                lfx -= fa * dxa;
+                                lfy -= fa * dya;
+                                lfz -= fa * dza;
+                                if (VK != 2 && (VK == 1 || myidx > rec_idx(wa))) {
+                                    le += ea;
+                                    lv -= fa * r2a;
+                                    lnp++;
+                                    lnref += (VK == 0 || !P.proj ||
+                                              (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                }
                             }
-                            if (cb) {
-                                le += eb;
-                                lv -= fb * r2b;
-                                lnp++;
-                                lnref += (self || (live_ref && ((int)rec_key(wb) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
-                            }
+                            wave_lds_sync();
                         }
-                    } else {
-                        /* Below the pair's r_lo (or no network): table, by this lane.
-                           Otherwise keep the hit for the dense list. */
-                        int nd = 0;
-                        for (int s = 0; s < h; s++) {
-                            const int ta = hitl[s * 64 + lane];
-                            const float4 pa = stage[ta];
-                            const unsigned int wa = __float_as_uint(pa.w);
-                            const float4 ha = ptab[ti * mt + rec_type(wa)];
-                            if (__float_as_int(ha.w) == -1) continue;
-                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
-                            const float r2a = dxa * dxa + dya * dya + dza * dza;
-                            const float rlo = N.sc[ptype[ti * mt + rec_type(wa)]].x;
-                            if (r2a >= rlo * rlo) {
-                                hitl[nd * 64 + lane] = (unsigned short)ta;
-                                nd++;
-                                continue;
-                            }
-                            float ea, fa;
-                            pot_eval32(ha, coeffs4, r2a, ea, fa);
-                            lfx -= fa * dxa;
-                            lfy -= fa * dya;
-                            lfz -= fa * dza;
-                            if (self ? (myidx > rec_idx(wa)) : (orient == 0)) {
-                                le += ea;
-                                lv -= fa * r2a;
-                                lnp++;
-                                lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
-                            }
-                        }
-                        /* Dense list: this lane's hits at [off, off + nd). */
-                        const int off = wave_excl_scan(nd, lane);
-                        const int T = __shfl(off + nd, 63);
-                        for (int s = 0; s < nd; s++) {
-                            const float4 pa = stage[hitl[s * 64 + lane]];
-                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
-                            dn_r2[off + s] = dxa * dxa + dya * dya + dza * dza;
-                            dn_p[off + s] = ptype[ti * mt + rec_type(__float_as_uint(pa.w))];
-                        }
-                        wave_lds_sync();
-                        mlp_dense_eval(N, dn_r2, dn_p, dn_f, dn_e, T, lane);
-                        wave_lds_sync();
-                        for (int s = 0; s < nd; s++) {
-                            const float4 pa = stage[hitl[s * 64 + lane]];
-                            const unsigned int wa = __float_as_uint(pa.w);
-                            const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1, dza = (xi.z - pa.z) - sf2;
-                            const float r2a = dxa * dxa + dya * dya + dza * dza;
-                            const float fa = dn_f[off + s], ea = dn_e[off + s];
-                            lfx -= fa * dxa;
-                            lfy -= fa * dya;
-                            lfz -= fa * dza;
-                            if (self ? (myidx > rec_idx(wa)) : (orient == 0)) {
-                                le += ea;
-                                lv -= fa * r2a;
-                                lnp++;
-                                lnref += (self || (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
-                            }
-                        }
-                        wave_lds_sync();
                     }
-                }
+                };
+                if (self)
+                    visit(visit_kind<0>{});
+                else if (orient == 0)
+                    visit(visit_kind<1>{});
+                else
+                    visit(visit_kind<2>{});
                 acc2(hx, lx, lfx);
                 acc2(hy, ly, lfy);
                 acc2(hz, lz, lfz);
