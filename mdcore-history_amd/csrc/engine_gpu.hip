@@ -161,12 +161,14 @@ __global__ __launch_bounds__(256) void k_clear_ids(int nids, int *__restrict__ i
 
 /* ------------------------------------------------------------ k_sort */
 
+#define MD_SORT_STACK 16
+
 /* Descending sort of (index<<16 | key) words by key: the reference's
    quicksort with selection sort below 16 elements and an explicit stack
    (runner.c:110-174).  The partition and push order decide the order of
    equal keys, so they are kept as they are. */
-__device__ void sort_desc(unsigned int *a, int N, int *overflow) {
-    int st_lo[16], st_hi[16];
+template <typename A, typename S>
+__device__ __forceinline__ void sort_desc_t(A a, int N, S st_lo, S st_hi, int *overflow) {
     int qpos = 0;
     st_lo[0] = 0;
     st_hi[0] = N - 1;
@@ -206,7 +208,7 @@ __device__ void sort_desc(unsigned int *a, int N, int *overflow) {
                     j -= 1;
                 }
             }
-            if (qpos + 2 >= 16) {
+            if (qpos + 2 >= MD_SORT_STACK) {
                 *overflow = 1;
                 return;
             }
@@ -219,6 +221,11 @@ __device__ void sort_desc(unsigned int *a, int N, int *overflow) {
             }
         }
     }
+}
+
+__device__ void sort_desc(unsigned int *a, int N, int *overflow) {
+    int st_lo[MD_SORT_STACK], st_hi[MD_SORT_STACK];
+    sort_desc_t(a, N, st_lo, st_hi, overflow);
 }
 
 __constant__ double c_axis[39];
@@ -251,6 +258,38 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
         keyof[(size_t)sid * P.nparts + off + i] = (unsigned short)key;
     }
     sort_desc(list, n, flag);
+}
+
+/* The same sort with each thread's segment and quicksort stack in a private
+   slice of LDS (stride words per thread, odd so that equal indices of
+   different threads fall in different banks): the global sort list is
+   written once, after sorting, instead of being swapped in place.  For
+   cells of at most MD_SORT_LDS_MAX particles. */
+#define MD_SORT_LDS_MAX 191 /* 64 threads x (191 + 32) words = 57 KB of LDS */
+__global__ __launch_bounds__(64) void k_sort_lds(struct nb_params P, const int *__restrict__ count,
+                                                 const int *__restrict__ offset,
+                                                 const unsigned int *__restrict__ sortflags,
+                                                 const double *__restrict__ x, unsigned int *__restrict__ sortlist,
+                                                 unsigned short *__restrict__ keyof, int *__restrict__ flag,
+                                                 int stride) {
+    extern __shared__ unsigned int sh_sort[];
+    unsigned int *list = sh_sort + threadIdx.x * stride;
+    int *stk = (int *)(sh_sort + 64 * stride) + threadIdx.x * 2 * MD_SORT_STACK;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int cid = t / 13, sid = t - 13 * (t / 13);
+    if (cid >= P.ncells) return;
+    if (!((sortflags[cid] >> sid) & 1u)) return;
+    int n = count[cid];
+    if (n == 0) return;
+    int off = offset[cid];
+    for (int i = 0; i < n; i++) {
+        unsigned int key = sort_key(&x[3 * (size_t)(off + i)], sid, P.dscale, P.bias);
+        list[i] = ((unsigned int)i << 16) | key;
+        keyof[(size_t)sid * P.nparts + off + i] = (unsigned short)key;
+    }
+    sort_desc_t(list, n, stk, stk + MD_SORT_STACK, flag);
+    unsigned int *out = sortlist + 13 * (size_t)off + (size_t)sid * n;
+    for (int i = 0; i < n; i++) out[i] = list[i];
 }
 
 /* ------------------------------------------------------------ k_nonbond */
@@ -1392,9 +1431,17 @@ static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     nb = ((g->nparts > g->ncells ? g->nparts : g->ncells) + 255) / 256;
     hipLaunchKernelGGL(k_pack, dim3(nb), dim3(256), 0, g->stream, g->nparts, g->d_x, g->d_type, g->d_id, g->ncells,
                        g->d_offset, g->d_xf, g->d_index_of_id, g->d_cell_of, g->d_queue);
-    nb = (g->ncells * 13 + 255) / 256;
-    hipLaunchKernelGGL(k_sort, dim3(nb), dim3(256), 0, g->stream, P, g->d_count, g->d_offset, g->d_sortflags,
-                       g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE);
+    if (g->maxcount <= MD_SORT_LDS_MAX) {
+        const int stride = g->maxcount | 1;
+        const size_t sh = sizeof(unsigned int) * 64 * ((size_t)stride + 2 * MD_SORT_STACK);
+        nb = (g->ncells * 13 + 63) / 64;
+        hipLaunchKernelGGL(k_sort_lds, dim3(nb), dim3(64), sh, g->stream, P, g->d_count, g->d_offset,
+                           g->d_sortflags, g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE, stride);
+    } else {
+        nb = (g->ncells * 13 + 255) / 256;
+        hipLaunchKernelGGL(k_sort, dim3(nb), dim3(256), 0, g->stream, P, g->d_count, g->d_offset, g->d_sortflags,
+                           g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE);
+    }
     if (timed) HIPCHECK(hipEventRecord(g->ev[1], g->stream));
     struct nb_out O;
     O.fhi = g->d_fhi;

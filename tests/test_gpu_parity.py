@@ -25,7 +25,18 @@ SYSTEMS = {
     "mixture_4k": lambda: systems.mixture(4000, seed=11),
     # ~100 atoms/cell (two 50-atom chunks per wave), exclusions, three tables
     "water_1k": lambda: systems.water(1000, 3.1, (10, 10, 10), seed=5),
+    # ~330 atoms/cell: six chunks per wave and the sort path for cells too
+    # large for the LDS sort
+    "dense_bigcells": lambda: dense_big_cells(),
 }
+
+
+def dense_big_cells(L=3.9, density=150.0, seed=3):
+    N = int(density * L ** 3)
+    x = md.gen_uniform(3 * N, 0.0, L, seed).reshape(-1, 3)
+    pot = md.LJ126(0.05, 1.2, 1e-6, 1e-3, 1e-3)
+    return systems.System("dense", np.zeros(3), np.full(3, L), np.full(3, 1.3), 1.2, x, np.ones(N, np.int32),
+                          np.array([0.0, 1.0]), [(1, 1, pot)])
 
 
 @pytest.fixture(scope="module", params=sorted(SYSTEMS))
