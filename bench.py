@@ -266,7 +266,7 @@ def matched_table(sysm, f_table, target, args):
     import mdcore as md
     import systems
     best = None
-    for tol in (2e-3, 4e-3, 1e-2, 3e-2):
+    for tol in (1.2e-3, 1.5e-3, 2e-3, 4e-3):
         s2 = systems.with_tolerance(sysm, tol)
         e2 = systems.build_engine(s2, md.engine_flag_gpu)
         e2.shuffle()

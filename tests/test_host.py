@@ -110,3 +110,39 @@ def test_setdt_rejects_negative():
         e.setdt(-1.0)
     md.library().errs_clear()
     e.close()
+
+
+def test_potential_eval_r_matches_scalar_eval():
+    p = md.LJ126_RF(0.12, 1.2, 2e-6, 2e-3, 0.3, 138.935458, 0.0, 1e-3)
+    r = np.linspace(0.1, 1.2, 257)
+    e, dedr = p.eval_r(r)
+    for k in range(0, 257, 16):
+        es, fs = p.eval(r[k] * r[k])
+        assert e[k] == es
+        assert dedr[k] == fs * r[k]
+
+
+def test_sample_pairs_finds_every_pair_once_per_side():
+    """All in-cutoff ordered pairs of a small periodic box, against numpy."""
+    rng = np.random.default_rng(5)
+    L, N, rc = 3.3, 300, 1.0
+    x = rng.random((N, 3)) * L
+    t = rng.integers(1, 4, N).astype(np.int32)
+    r, ti, tj = md.sample_pairs(x, t, [L] * 3, rc, 10 ** 6, 7)
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    want = np.sort(np.sqrt(r2[r2 <= rc * rc]))
+    assert len(r) == len(want)
+    assert np.allclose(np.sort(r), want, rtol=0, atol=1e-12)
+    # types follow the pairs: counts of (ti, tj) combinations agree
+    ii, jj = np.nonzero(r2 <= rc * rc)
+    got = np.bincount(ti * 4 + tj, minlength=16)
+    ref = np.bincount(t[ii] * 4 + t[jj], minlength=16)
+    assert np.array_equal(got, ref)
+
+
+def test_sample_pairs_rejects_small_box():
+    with pytest.raises(md.MdcoreError):
+        md.sample_pairs(np.zeros((4, 3)), np.ones(4, np.int32), [2.0] * 3, 1.0, 10, 1)
