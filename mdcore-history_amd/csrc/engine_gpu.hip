@@ -112,6 +112,47 @@ __device__ __forceinline__ void pot_eval32(const float4 hdr, const float4 *__res
     f = eff * a.y * ir;
 }
 
+/* Buffer-resource loads: 32-bit byte offsets instead of 64-bit addresses
+   (word 3 of the descriptor as for every gfx9 target); an offset past the
+   declared size reads zeros. */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, int byte_off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+    float4 out;
+    __builtin_memcpy(&out, &v, sizeof(out));
+    return out;
+}
+
+/* pot_eval32 with the coefficients behind a buffer resource. */
+__device__ __forceinline__ void pot_eval32r(const float4 hdr, __amdgpu_buffer_rsrc_t rc, float r2, float &e,
+                                            float &f) {
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ir;
+    const float fi = fmaxf(0.0f, hdr.x + r * (hdr.y + r * hdr.z));
+    const unsigned int w = __float_as_uint(hdr.w);
+    const int n = (int)(w & 511u);
+    int ind = (int)fi;
+    ind = ind > n ? n : ind;
+    const int off = 32 * ((int)(w >> 9) + ind);
+    const float4 a = ld_f4(rc, off), b = ld_f4(rc, off + 16);
+    const float x = (r - a.x) * a.y;
+    float ee = a.z * x + a.w;
+    float eff = a.z;
+    eff = eff * x + ee;
+    ee = ee * x + b.x;
+    eff = eff * x + ee;
+    ee = ee * x + b.y;
+    eff = eff * x + ee;
+    ee = ee * x + b.z;
+    eff = eff * x + ee;
+    ee = ee * x + b.w;
+    e = ee;
+    f = eff * a.y * ir;
+}
+
 /* The pair kernel's workgroup is a single wavefront, and the LDS executes a
    wavefront's instructions in order: data written by one lane is visible to
    another lane's later read once this wave's LDS operations have completed.
@@ -359,7 +400,8 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
 template <bool MLP>
 __device__ __forceinline__ void nonbond_body(
     unsigned char *smem, struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
-    const int *__restrict__ nbr_off, const double *__restrict__ origin, const int *__restrict__ loc,
+    const int *__restrict__ nbr_off, const float4 *__restrict__ nbs, const double *__restrict__ origin,
+    const int *__restrict__ loc,
     const float4 *__restrict__ xf, const double *__restrict__ x64, const int *__restrict__ pid,
     const unsigned int *__restrict__ sortlist, const unsigned short *__restrict__ keyof,
     const float4 *__restrict__ ptab, const float4 *__restrict__ coeffs4, const int *__restrict__ excl_off,
@@ -369,8 +411,6 @@ __device__ __forceinline__ void nonbond_body(
     float4 *raw = (float4 *)smem;                                        /* [64]        */
     unsigned short *own_key = (unsigned short *)(smem + P.lds_own_key);  /* [13*64]     */
     int *kfl = (int *)(smem + P.lds_kfl);                                /* [26]        */
-    int4 *s_nbe = (int4 *)(smem + P.lds_nbe);                            /* [maxnbr]    */
-    int2 *s_nbc = (int2 *)(smem + P.lds_nbc);                            /* [maxnbr]    */
     float4 *stage = (float4 *)(smem + P.lds_stage);                      /* [maxn]      */
     unsigned short *hitl = (unsigned short *)(smem + P.lds_hit);         /* [HCAP*64]   */
     /* MLP mode: dense pair list (r^2, species pair) and its results (f, e),
@@ -382,6 +422,8 @@ __device__ __forceinline__ void nonbond_body(
     int *exb = (int *)(smem + P.lds_dense + 16 * 64 * MD_HCAP_MLP);      /* [XCAP*64]   */
     const int lane = threadIdx.x;
     const int mt = P.max_type;
+    const __amdgpu_buffer_rsrc_t rs_ptab = make_rsrc(ptab, P.ptab_bytes);
+    const __amdgpu_buffer_rsrc_t rs_coef = make_rsrc(coeffs4, P.coef_bytes);
 
     int q = blockIdx.x % MD_NQUEUE, tried = 0;
     while (tried < MD_NQUEUE) {
@@ -395,20 +437,23 @@ __device__ __forceinline__ void nonbond_body(
             tried++;
             continue;
         }
-        const int c = lo_q + it;
+        /* Wave-uniform from here: the neighbour list is read with scalar loads. */
+        const int c = __builtin_amdgcn_readfirstlane(lo_q + it);
         const int2 co_c = cnt_off[c];
         const int nc = co_c.x, offc = co_c.y;
-        if (nc == 0) continue;
+        if (nc == 0) {
+            /* An empty cell still owns its slots of the per-cell sums. */
+            if (lane < 3) {
+                O.cell_acc[3 * c + lane] = 0.0;
+                O.cell_cnt[3 * c + lane] = 0ull;
+            }
+            continue;
+        }
         const double oc0 = origin[3 * c], oc1 = origin[3 * c + 1], oc2 = origin[3 * c + 2];
         const int k0 = nbr_off[c], nn = nbr_off[c + 1] - k0;
 
-        /* Owner metadata: neighbour list, first/last key per direction. */
+        /* Owner metadata: first/last key per direction. */
         wave_lds_sync();
-        for (int k = lane; k < nn; k += 64) {
-            const int4 en = nbr[k0 + k];
-            s_nbe[k] = en;
-            s_nbc[k] = cnt_off[en.x];
-        }
         if (lane < 26) {
             const int sid = lane % 13;
             kfl[lane] = (int)(sortlist[13 * (size_t)offc + (size_t)sid * nc + (lane < 13 ? 0 : nc - 1)] & 0xffffu);
@@ -440,8 +485,8 @@ __device__ __forceinline__ void nonbond_body(
             float4 pfx = make_float4(0.f, 0.f, 0.f, 0.f);
             unsigned int pfw = 0;
             {
-                const int4 en = s_nbe[0];
-                const int2 cj = s_nbc[0];
+                const int4 en = nbr[k0];
+                const int2 cj = cnt_off[en.x];
                 if (lane < cj.x) {
                     pfx = xf[cj.y + lane];
                     if (!(en.y & 32)) {
@@ -456,8 +501,8 @@ __device__ __forceinline__ void nonbond_body(
             unsigned int np = 0, nref = 0, nexcl = 0;
 
             for (int k = 0; k < nn; k++) {
-                const int4 en = s_nbe[k];
-                const int2 cjc = s_nbc[k];
+                const int4 en = nbr[k0 + k];
+                const int2 cjc = cnt_off[en.x];
                 const int cj = en.x, nj = cjc.x, offj = cjc.y;
                 const int sid = en.y & 15, orient = (en.y >> 4) & 1, self = (en.y >> 5) & 1;
 
@@ -504,8 +549,8 @@ __device__ __forceinline__ void nonbond_body(
 
                 /* Prefetch the next neighbour while this one is processed. */
                 if (k + 1 < nn) {
-                    const int4 en1 = s_nbe[k + 1];
-                    const int2 c1 = s_nbc[k + 1];
+                    const int4 en1 = nbr[k0 + k + 1];
+                    const int2 c1 = cnt_off[en1.x];
                     if (lane < c1.x) {
                         pfx = xf[c1.y + lane];
                         if (!(en1.y & 32)) {
@@ -517,10 +562,10 @@ __device__ __forceinline__ void nonbond_body(
                 if (nj == 0) continue;
 
                 /* Shift from the owner cell to the neighbour (minimum image),
-                   rounded to FP32; the FP64 value is recomputed where needed. */
-                const float sf0 = (float)min_image(origin[3 * cj] - oc0, P.dim[0]);
-                const float sf1 = (float)min_image(origin[3 * cj + 1] - oc1, P.dim[1]);
-                const float sf2 = (float)min_image(origin[3 * cj + 2] - oc2, P.dim[2]);
+                   rounded to FP32 on the host; the FP64 value is recomputed
+                   where needed. */
+                const float4 sfv = nbs[k0 + k];
+                const float sf0 = sfv.x, sf1 = sfv.y, sf2 = sfv.z;
 
                 int myk = 0, dns = 0;
                 bool live_ref = true, run = true;
@@ -599,7 +644,8 @@ __device__ __forceinline__ void nonbond_body(
                                 const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
                                 const float4 pa = stage[ta], pb = stage[tb];
                                 const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
-                                const float4 ha = ptab[ti * mt + rec_type(wa)], hb = ptab[ti * mt + rec_type(wb)];
+                                const float4 ha = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wa)));
+                                const float4 hb = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wb)));
                                 const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
                                             dza = (xi.z - pa.z) - sf2;
                                 const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1,
@@ -609,8 +655,8 @@ __device__ __forceinline__ void nonbond_body(
                                 const bool ua = va && __float_as_int(ha.w) != -1;
                                 const bool ub = vb && __float_as_int(hb.w) != -1;
                                 float ea = 0.f, fa = 0.f, eb = 0.f, fb = 0.f;
-                                if (ua) pot_eval32(ha, coeffs4, r2a, ea, fa);
-                                if (ub) pot_eval32(hb, coeffs4, r2b, eb, fb);
+                                if (ua) pot_eval32r(ha, rs_coef, r2a, ea, fa);
+                                if (ub) pot_eval32r(hb, rs_coef, r2b, eb, fb);
                                 lfx -= fa * dxa + fb * dxb;
                                 lfy -= fa * dya + fb * dyb;
                                 lfz -= fa * dza + fb * dzb;
@@ -887,14 +933,15 @@ __device__ __forceinline__ void nonbond_body(
 
 #define NB_PARAMS                                                                                             \
     struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,                       \
-        const int *__restrict__ nbr_off, const double *__restrict__ origin, const int *__restrict__ loc,      \
+        const int *__restrict__ nbr_off, const float4 *__restrict__ nbs, const double *__restrict__ origin,   \
+        const int *__restrict__ loc,                                                                         \
         const float4 *__restrict__ xf, const double *__restrict__ x64, const int *__restrict__ pid,          \
         const unsigned int *__restrict__ sortlist, const unsigned short *__restrict__ keyof,                \
         const float4 *__restrict__ ptab, const float4 *__restrict__ coeffs4, const int *__restrict__ excl_off, \
         const int *__restrict__ excl_list, const int *__restrict__ index_of_id, const int *__restrict__ cell_of, \
         struct nb_out O, int *__restrict__ queue
 #define NB_ARGS                                                                                              \
-    P, cnt_off, nbr, nbr_off, origin, loc, xf, x64, pid, sortlist, keyof, ptab, coeffs4, excl_off, excl_list, \
+    P, cnt_off, nbr, nbr_off, nbs, origin, loc, xf, x64, pid, sortlist, keyof, ptab, coeffs4, excl_off, excl_list, \
         index_of_id, cell_of, O, queue
 
 /* Table mode.  The occupancy hint keeps it at 4 waves per SIMD (128
@@ -1042,6 +1089,7 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     free(g->h_pair_ti);
     free(g->h_pair_tj);
     (void)hipFree(g->d_nbr);
+    (void)hipFree(g->d_nbs);
     (void)hipFree(g->d_nbr_off);
     (void)hipFree(g->d_origin);
     (void)hipFree(g->d_loc);
@@ -1065,6 +1113,21 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     free(g);
     e->gpu = NULL;
     return engine_err_ok;
+}
+
+/* FP32 minimum-image shift from cell a's origin to cell b's: the same FP64
+   arithmetic as min_image() on the device, rounded once. */
+static float4 cell_shift32(const struct space *s, int a, int b) {
+    float v[3];
+    for (int k = 0; k < 3; k++) {
+        double d = s->cells[b].origin[k] - s->cells[a].origin[k];
+        if (d * 2 > s->dim[k])
+            d -= s->dim[k];
+        else if (d * 2 < -s->dim[k])
+            d += s->dim[k];
+        v[k] = (float)d;
+    }
+    return make_float4(v[0], v[1], v[2], 0.f);
 }
 
 /* dnshift of a pair task (runner_dopair.c:426-452): with the reference's
@@ -1189,6 +1252,8 @@ extern "C" int engine_gpu_load(struct engine *e) {
     {
         struct nb_params *P = &g->P;
         memset(P, 0, sizeof(*P));
+        P->ptab_bytes = (int)(sizeof(float4) * mt * mt);
+        P->coef_bytes = (int)(sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1));
         P->ncells = s->nr_cells;
         P->max_type = mt;
         P->proj = (e->flags & engine_flag_projwindow) ? 1 : 0;
@@ -1212,6 +1277,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
         int *cnt = (int *)calloc(s->nr_cells + 1, sizeof(int));
         int *fill, *off;
         int4 *nbr;
+        float4 *nbs;
         int total = 0, maxnbr = 1;
         for (k = 0; k < s->nr_tasks; k++) {
             struct task *t = &s->tasks[k];
@@ -1232,14 +1298,18 @@ extern "C" int engine_gpu_load(struct engine *e) {
         fill[0] = 0;
         for (cid = 0; cid < s->nr_cells; cid++) fill[cid + 1] = fill[cid] + cnt[cid];
         memcpy(off, fill, sizeof(int) * (s->nr_cells + 1));
+        nbs = (float4 *)malloc(sizeof(float4) * (total > 0 ? total : 1));
         for (k = 0; k < s->nr_tasks; k++) {
             struct task *t = &s->tasks[k];
             if (t->type == task_type_self) {
+                nbs[fill[t->i]] = make_float4(0.f, 0.f, 0.f, 0.f);
                 nbr[fill[t->i]++] = make_int4(t->i, 1 << 5, 0, 0);
             } else if (t->type == task_type_pair) {
                 int dr, dp;
                 task_dnshift(s, t->i, t->j, t->flags, g->P.dscale, &dr, &dp);
+                nbs[fill[t->i]] = cell_shift32(s, t->i, t->j);
                 nbr[fill[t->i]++] = make_int4(t->j, t->flags, dr, dp);
+                nbs[fill[t->j]] = cell_shift32(s, t->j, t->i);
                 nbr[fill[t->j]++] = make_int4(t->i, t->flags | (1 << 4), dr, dp);
             }
         }
@@ -1248,11 +1318,14 @@ extern "C" int engine_gpu_load(struct engine *e) {
         HIPCHECK(hipMalloc(&g->d_nbr, sizeof(int4) * (total > 0 ? total : 1)));
         HIPCHECK(hipMalloc(&g->d_nbr_off, sizeof(int) * (s->nr_cells + 1)));
         HIPCHECK(hipMemcpy(g->d_nbr, nbr, sizeof(int4) * (total > 0 ? total : 1), hipMemcpyHostToDevice));
+        HIPCHECK(hipMalloc(&g->d_nbs, sizeof(float4) * (total > 0 ? total : 1)));
+        HIPCHECK(hipMemcpy(g->d_nbs, nbs, sizeof(float4) * (total > 0 ? total : 1), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(g->d_nbr_off, off, sizeof(int) * (s->nr_cells + 1), hipMemcpyHostToDevice));
         free(cnt);
         free(fill);
         free(off);
         free(nbr);
+        free(nbs);
     }
 
     /* Cell origins, locations and sort masks. */
@@ -1399,10 +1472,6 @@ static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     lds = (lds + 15) & ~(size_t)15;
     P.lds_kfl = (int)lds;
     lds += sizeof(int) * 32;
-    P.lds_nbe = (int)lds;
-    lds += sizeof(int4) * (size_t)g->maxnbr;
-    P.lds_nbc = (int)lds;
-    lds += sizeof(int2) * (size_t)g->maxnbr;
     lds = (lds + 15) & ~(size_t)15;
     P.lds_stage = (int)lds;
     lds += sizeof(float4) * maxn;
@@ -1450,12 +1519,12 @@ static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     O.cell_cnt = g->d_cell_cnt;
     if (mlp)
         hipLaunchKernelGGL(k_nonbond_mlp, dim3(g->grid_nb[1]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
-                           g->d_nbr_off, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
+                           g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue, gpu_mlp_net(g), g->d_ptype);
     else
         hipLaunchKernelGGL(k_nonbond, dim3(g->grid_nb[0]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
-                           g->d_nbr_off, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
+                           g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue);
     if (timed) HIPCHECK(hipEventRecord(g->ev[2], g->stream));

@@ -34,8 +34,9 @@ struct nb_params {
     double dim[3], h[3];
     float rc2f, band;
     int maxn, maxnbr;
+    int ptab_bytes, coef_bytes; /* sizes of the table headers and coefficients */
     /* dynamic LDS layout (byte offsets) */
-    int lds_own_key, lds_kfl, lds_nbe, lds_nbc, lds_stage, lds_hit, lds_dense;
+    int lds_own_key, lds_kfl, lds_stage, lds_hit, lds_dense;
 };
 
 /* Neighbour-list entry of an owner cell: x = other cell, y = sid | orient<<4
@@ -57,6 +58,7 @@ struct gpu_state {
     float4 *d_ptab;
     float *d_coeffs;
     int4 *d_nbr;
+    float4 *d_nbs; /* per neighbour entry: FP32 minimum-image shift owner -> neighbour */
     int *d_nbr_off;
     double *d_origin;
     int *d_loc;
