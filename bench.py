@@ -38,6 +38,7 @@ FLOP_PER_PAIR = 46          # algorithmic FLOP per evaluated pair interaction
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 peak
 MLP_MFMA_FLOP = 2 * 64 * 64  # 64x64 layer on MFMA per network evaluation
 BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
+MI355X_CUS = 256            # the pair kernel's 5-wave build runs from 64 cells per CU up
 DT_PS = 0.002               # 2 fs time step for ns/day
 METRIC = "pair-interactions/s & ns/day at 1M atoms; MLP vs table at fixed force RMSE"
 TRAFFIC_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_traffic.json"
@@ -142,7 +143,8 @@ def main():
             traffic = None
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_nonbond", "kernel_ms": round(ms["nonbond_ms"], 4),
+                "kernel": "k_nonbond_wide" if eng_cells(sysm) >= 64 * MI355X_CUS else "k_nonbond",
+                "kernel_ms": round(ms["nonbond_ms"], 4),
                 "note": "FP32 vector pipe (VALU); packed FP32 disabled, so 78.65 TFLOP/s is the issue ceiling"}
 
     out = {

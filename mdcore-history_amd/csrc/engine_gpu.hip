@@ -953,6 +953,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     nonbond_body<false>(smem, NB_ARGS, N, nullptr);
 }
 
+/* Table mode for systems with many more cells than resident waves: the same
+   body under a 96-register budget (5 waves per SIMD), which pays off when
+   every wave has cells to take; with few cells per wave the 4-wave build's
+   lower latency per cell wins. */
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_wide(NB_PARAMS) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const struct mlp_net N = {};
+    nonbond_body<false>(smem, NB_ARGS, N, nullptr);
+}
+
 /* Surrogate mode (engine_flag_mlp). */
 __global__ __launch_bounds__(64) void k_nonbond_mlp(NB_PARAMS, struct mlp_net N, const int *__restrict__ ptype) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -1406,7 +1416,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
         HIPCHECK(hipGetDeviceProperties(&prop, g->device));
         g->ncu = prop.multiProcessorCount;
     }
-    g->lds_nb[0] = g->lds_nb[1] = 0;
+    g->lds_nb[0] = g->lds_nb[1] = g->lds_nb[2] = 0;
     return engine_err_ok;
 }
 
@@ -1462,6 +1472,8 @@ static int gpu_upload(struct engine *e) {
 /* One device pass on the resident particles; mlp selects the surrogate
    mode of the pair kernel. */
 static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
+    /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget */
+    const int kern = mlp ? 1 : (g->ncells >= 4 * 16 * g->ncu ? 2 : 0);
     struct nb_params P = g->P;
     int nb;
     size_t lds = 0;
@@ -1481,16 +1493,18 @@ static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     P.lds_dense = (int)lds;
     if (mlp) lds += 16 * 64 * MD_HCAP_MLP + 4 * 64 * MD_XCAP_MLP;
     if (lds > 64 * 1024) return gerror_(engine_err_maxparts, "LDS staging exceeds 64 KiB");
-    if (lds != g->lds_nb[mlp]) {
+    if (lds != g->lds_nb[kern]) {
         /* Persistent grid: as many single-wave blocks as can be resident. */
         int per_cu = 0;
-        if (mlp)
+        if (kern == 1)
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_mlp, 64, lds));
+        else if (kern == 2)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_wide, 64, lds));
         else
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond, 64, lds));
         if (per_cu < 1) per_cu = 1;
-        g->grid_nb[mlp] = g->ncu * per_cu;
-        g->lds_nb[mlp] = lds;
+        g->grid_nb[kern] = g->ncu * per_cu;
+        g->lds_nb[kern] = lds;
     }
     if (mlp && g->mlp_np == 0) return gerror_(engine_err_range, "engine_flag_mlp without engine_mlp_init");
 
@@ -1517,13 +1531,14 @@ static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     O.flo = g->d_flo;
     O.cell_acc = g->d_cell_acc;
     O.cell_cnt = g->d_cell_cnt;
-    if (mlp)
+    if (kern == 1)
         hipLaunchKernelGGL(k_nonbond_mlp, dim3(g->grid_nb[1]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue, gpu_mlp_net(g), g->d_ptype);
     else
-        hipLaunchKernelGGL(k_nonbond, dim3(g->grid_nb[0]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
+        hipLaunchKernelGGL(kern == 2 ? k_nonbond_wide : k_nonbond, dim3(g->grid_nb[kern]), dim3(64), lds, g->stream,
+                           P, g->d_cnt_off, g->d_nbr,
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue);

@@ -52,8 +52,8 @@ struct gpu_state {
     int ncells, ntab, max_type, nparts_cap, nids_cap, maxcount, maxnbr;
     int nparts, nids;
     struct nb_params P;
-    int grid_nb[2], ncu; /* [0] table mode, [1] surrogate mode */
-    size_t lds_nb[2];
+    int grid_nb[3], ncu; /* [0] table mode, [1] surrogate mode, [2] table mode, 5-wave budget */
+    size_t lds_nb[3];
     /* static */
     float4 *d_ptab;
     float *d_coeffs;
