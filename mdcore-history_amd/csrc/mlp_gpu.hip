@@ -25,6 +25,9 @@
 
 #include "gpu_state.h"
 
+#define MLP_WPART (MLP_H * MLP_H + 3 * MLP_H) /* per-wave partial: W2, W1, b2 */
+#define MLP_GRID_MAX 2048
+
 /* ------------------------------------------------------------ kernels */
 
 __global__ __launch_bounds__(256) void k_mlp_fill(float *__restrict__ a, long long n, float v) {
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
                                                  const float *__restrict__ tF, const int *__restrict__ tiles,
                                                  int ntiles, float wE, float wF, float inv_n, int np,
                                                  float *__restrict__ grad, float *__restrict__ gb1e,
-                                                 float *__restrict__ loss) {
+                                                 float *__restrict__ loss, float *__restrict__ partial) {
     __shared__ __attribute__((aligned(16))) __bf16 h1t[MLP_H][32]; /* [in unit][sample of the tile pair] */
     __shared__ __attribute__((aligned(16))) __bf16 dzt[MLP_H][32]; /* [out unit][sample]               */
     const int lane = threadIdx.x, n = lane & 15, g = lane >> 4;
@@ -177,45 +180,61 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
         __syncthreads();
     }
 
-    /* Flush this wave's sums. */
+    /* This wave's sums of the shared weights go to its own partial row
+       (summed over waves in a fixed order by k_mlp_wsum): W2, then W1, b2. */
+    float *row = partial + (size_t)blockIdx.x * MLP_WPART;
 #pragma unroll
     for (int mb = 0; mb < 4; mb++)
 #pragma unroll
         for (int kb = 0; kb < 4; kb++)
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                atomicAdd(&grad[MLP_OFF_W2 + (16 * mb + 4 * g + i) * MLP_H + 16 * kb + n], dw2[mb * 4 + kb][i]);
+            for (int i = 0; i < 4; i++) row[(16 * mb + 4 * g + i) * MLP_H + 16 * kb + n] = dw2[mb * 4 + kb][i];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int u = mlp_unit(j, lane);
         const float a = mlp_sum16(dw1a[j]), b = mlp_sum16(dw1b[j]), c = mlp_sum16(db2[j]);
         if (n == 0) {
-            atomicAdd(&grad[MLP_OFF_W1 + 2 * u], a);
-            atomicAdd(&grad[MLP_OFF_W1 + 2 * u + 1], b);
-            atomicAdd(&grad[MLP_OFF_B2 + u], c);
+            row[MLP_H * MLP_H + 2 * u] = a;
+            row[MLP_H * MLP_H + 2 * u + 1] = b;
+            row[MLP_H * MLP_H + 2 * MLP_H + u] = c;
         }
     }
     lsum = mlp_sum16(lsum);
     if (lane == 0) atomicAdd(loss, lsum * inv_n);
 }
 
-/* b1e[p][u] = b1[u] + sum_e E1[u][e] emb[p][e]: gradients of b1, E1, emb. */
+/* Sum the per-wave partial rows of k_mlp_grad into the gradient, wave by
+   wave in order. */
+__global__ __launch_bounds__(256) void k_mlp_wsum(const float *__restrict__ partial, int nwaves,
+                                                  float *__restrict__ grad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= MLP_WPART) return;
+    float acc = 0.f;
+    for (int w = 0; w < nwaves; w++) acc += partial[(size_t)w * MLP_WPART + t];
+    if (t < MLP_H * MLP_H)
+        grad[MLP_OFF_W2 + t] += acc;
+    else if (t < MLP_H * MLP_H + 2 * MLP_H)
+        grad[MLP_OFF_W1 + t - MLP_H * MLP_H] += acc;
+    else
+        grad[MLP_OFF_B2 + t - MLP_H * MLP_H - 2 * MLP_H] += acc;
+}
+
+/* b1e[p][u] = b1[u] + sum_e E1[u][e] emb[p][e]: gradients of b1, E1, emb.
+   One thread per output gradient, each a fixed-order sum. */
 __global__ __launch_bounds__(256) void k_mlp_chain(const float *__restrict__ theta, int np,
                                                    const float *__restrict__ gb1e, float *__restrict__ grad) {
-    const int t = threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < MLP_H) {
-        float b = 0.f, e1[MLP_EMB];
-        for (int e = 0; e < MLP_EMB; e++) e1[e] = 0.f;
-        for (int p = 0; p < np; p++) {
-            const float gg = gb1e[p * MLP_H + t];
-            b += gg;
-            for (int e = 0; e < MLP_EMB; e++) e1[e] += gg * theta[MLP_OFF_EMB + p * MLP_EMB + e];
-        }
+        float b = 0.f;
+        for (int p = 0; p < np; p++) b += gb1e[p * MLP_H + t];
         grad[MLP_OFF_B1 + t] += b;
-        for (int e = 0; e < MLP_EMB; e++) grad[MLP_OFF_E1 + t * MLP_EMB + e] += e1[e];
-    }
-    for (int q = t; q < np * MLP_EMB; q += blockDim.x) {
-        const int p = q / MLP_EMB, e = q % MLP_EMB;
+    } else if (t < MLP_H + MLP_H * MLP_EMB) {
+        const int q = t - MLP_H, u = q / MLP_EMB, e = q % MLP_EMB;
+        float acc = 0.f;
+        for (int p = 0; p < np; p++) acc += gb1e[p * MLP_H + u] * theta[MLP_OFF_EMB + p * MLP_EMB + e];
+        grad[MLP_OFF_E1 + q] += acc;
+    } else if (t < MLP_H + MLP_H * MLP_EMB + np * MLP_EMB) {
+        const int q = t - MLP_H - MLP_H * MLP_EMB, p = q / MLP_EMB, e = q % MLP_EMB;
         float acc = 0.f;
         for (int u = 0; u < MLP_H; u++) acc += gb1e[p * MLP_H + u] * theta[MLP_OFF_E1 + u * MLP_EMB + e];
         grad[MLP_OFF_EMB + q] += acc;
@@ -281,6 +300,8 @@ void gpu_mlp_free(struct gpu_state *g) {
     (void)hipFree(g->d_w2f);
     (void)hipFree(g->d_w2b);
     (void)hipFree(g->d_msc);
+    (void)hipFree(g->d_wpart);
+    g->d_wpart = NULL;
     (void)hipFree(g->d_sr);
     (void)hipFree(g->d_sp);
     (void)hipFree(g->d_sE);
@@ -401,6 +422,7 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
         hipMalloc(&g->d_w2f, sizeof(mlp_bf16x8) * 512) != hipSuccess ||
         hipMalloc(&g->d_w2b, sizeof(mlp_bf16x8) * 512) != hipSuccess ||
         hipMalloc(&g->d_msc, sizeof(float4) * np) != hipSuccess ||
+        hipMalloc(&g->d_wpart, sizeof(float) * MLP_WPART * MLP_GRID_MAX) != hipSuccess ||
         hipMemcpy(g->d_theta, theta, sizeof(float) * nparams, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(g->d_msc, sc, sizeof(float4) * np, hipMemcpyHostToDevice) != hipSuccess) {
         free(theta);
@@ -446,13 +468,17 @@ static int mlp_check(const struct gpu_state *g, const float *r, const int *pair,
 static int mlp_gradient(struct gpu_state *g, const int *tiles, int ntiles, float wE, float wF, float *loss) {
     const int np = g->mlp_np;
     const int ntp = (ntiles + 1) / 2;
-    const int grid = ntp < 2048 ? ntp : 2048;
+    const int grid = ntp < MLP_GRID_MAX ? ntp : MLP_GRID_MAX;
     if (ntiles < 1) return gerror_(engine_err_range, "empty batch");
     hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_grad, (long long)g->mlp_nparams, 0.0f);
     hipLaunchKernelGGL(k_mlp_fill, dim3(16), dim3(256), 0, g->stream, g->d_gb1e, (long long)np * MLP_H, 0.0f);
     hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64), 0, g->stream, gpu_mlp_net(g), g->d_sr, g->d_sp, g->d_sE,
-                       g->d_sF, tiles, ntiles, wE, wF, 1.0f / (16.0f * ntiles), np, g->d_grad, g->d_gb1e, loss);
-    hipLaunchKernelGGL(k_mlp_chain, dim3(1), dim3(256), 0, g->stream, g->d_theta, np, g->d_gb1e, g->d_grad);
+                       g->d_sF, tiles, ntiles, wE, wF, 1.0f / (16.0f * ntiles), np, g->d_grad, g->d_gb1e, loss,
+                       g->d_wpart);
+    hipLaunchKernelGGL(k_mlp_wsum, dim3((MLP_WPART + 255) / 256), dim3(256), 0, g->stream, g->d_wpart, grid,
+                       g->d_grad);
+    hipLaunchKernelGGL(k_mlp_chain, dim3((MLP_H + MLP_H * MLP_EMB + np * MLP_EMB + 255) / 256), dim3(256), 0,
+                       g->stream, g->d_theta, np, g->d_gb1e, g->d_grad);
     HIPCHECK(hipGetLastError());
     return engine_err_ok;
 }
