@@ -72,6 +72,20 @@ int engine_gpu_time(struct engine *e, int nsteps, int nwarm, double *ms);
    engine_flag_projwindow is set). */
 int engine_gpu_results(struct engine *e, double *out, long long *counts);
 
+/* nsteps calls of engine_step (engine.c:2070-2240: engine_shuffle at 2101,
+   the nonbonded pass, engine_advance at 2191 and 1945-2052 with
+   space_shuffle_local, space.c:307-364) with the particles resident on the
+   device.  Positions, velocities, the per-cell particle order, the last
+   pass's forces and the energies left in e->s are bitwise those of the
+   host-driven loop; e->time advances by nsteps.  epot[k] (may be NULL)
+   receives the pass energy of step k (pairs + exclusions, what e->s.epot
+   holds after that engine_step) and ekin[k] (may be NULL) the kinetic
+   energy sum 1/2 m v^2 after its kick, summed per cell in FP64.  No
+   reference counterpart: the reference's CUDA path returns to the host
+   every step (runner_cuda.cu:2405-2430, 2514-2521).  As in engine_step of
+   this library, the nonbonded forces are the only forces integrated. */
+int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
+
 /* ------------------------------------------------------------------------
    MLP surrogate of the pair potential.  No reference counterpart: the
    reference evaluates only its piecewise-polynomial tables

@@ -278,6 +278,12 @@ int space_prepare(struct space *s);
 int space_shuffle_local(struct space *s);
 int space_getpos(struct space *s, int id, double *x);
 int space_flush(struct space *s);
+/* New helper: replace the cell contents by a layout built elsewhere (the
+   device step).  count[cid] particles per cell in storage order, given by
+   id with their new cell-relative x, v and type; charge, vid and flags
+   follow the id.  Forces are left to the caller. */
+int space_setcells(struct space *s, const int *count, const double *x, const double *v, const int *id,
+                   const int *type);
 void space_free(struct space *s);
 
 /* ------------------------------------------------------------------ exclusion */

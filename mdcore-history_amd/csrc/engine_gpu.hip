@@ -1421,7 +1421,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
 }
 
 /* Pack host particles in cell order and upload them (no kernels). */
-static int gpu_upload(struct engine *e) {
+int gpu_upload(struct engine *e) {
     struct gpu_state *g = (struct gpu_state *)e->gpu;
     struct space *s = &e->s;
     int cid, k, n = 0, maxc = 0, maxid = -1;
@@ -1471,7 +1471,7 @@ static int gpu_upload(struct engine *e) {
 /* Launch the kernels of one pass; optional events around each. */
 /* One device pass on the resident particles; mlp selects the surrogate
    mode of the pair kernel. */
-static int gpu_launch(struct gpu_state *g, int timed, int mlp) {
+int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget */
     const int kern = mlp ? 1 : (g->ncells >= 4 * 16 * g->ncu ? 2 : 0);
     struct nb_params P = g->P;

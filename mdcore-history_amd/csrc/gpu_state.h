@@ -113,6 +113,12 @@ static inline struct mlp_net gpu_mlp_net(const struct gpu_state *g) {
 
 void gpu_mlp_free(struct gpu_state *g);
 
+/* engine_gpu.hip: pack the host cells and upload them (no kernels); launch
+   one pass on the resident particles (mlp: surrogate mode), with events
+   around each kernel when timed. */
+int gpu_upload(struct engine *e);
+int gpu_launch(struct gpu_state *g, int timed, int mlp);
+
 #endif /* GPU_STATE_H */
 
 

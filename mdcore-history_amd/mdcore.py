@@ -90,6 +90,7 @@ def library():
         "engine_gpu_finalize": (C.c_int, [C.c_void_p]),
         "engine_gpu_time": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
         "engine_gpu_results": (C.c_int, [C.c_void_p, _dp, _llp]),
+        "engine_gpu_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
         "engine_mlp_pairs": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int]),
         "engine_mlp_init": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_ulonglong]),
         "engine_mlp_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
@@ -323,6 +324,14 @@ class Engine:
         ms = np.zeros(4)
         check(self.lib.engine_gpu_time(self.e, steps, warmup, ptr(ms)), "engine_gpu_time")
         return {"sort_ms": ms[0], "nonbond_ms": ms[1], "reduce_ms": ms[2], "total_ms": ms[3]}
+
+    def gpu_steps(self, nsteps):
+        """nsteps engine_step calls with the particles resident on the device
+        (engine_gpu_steps); returns the per-step pass energy and kinetic energy."""
+        epot = np.zeros(max(nsteps, 1))
+        ekin = np.zeros(max(nsteps, 1))
+        check(self.lib.engine_gpu_steps(self.e, nsteps, ptr(epot), ptr(ekin)), "engine_gpu_steps")
+        return epot[:nsteps], ekin[:nsteps]
 
     def cells(self, ncells):
         counts = np.zeros(ncells, dtype=np.int32)

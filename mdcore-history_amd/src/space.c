@@ -413,6 +413,50 @@ int space_flush(struct space *s) {
     return space_err_ok;
 }
 
+int space_setcells(struct space *s, const int *count, const double *x, const double *v, const int *id,
+                   const int *type) {
+    struct part *save;
+    int cid, k, m, a, n = 0;
+    if (s == NULL || count == NULL || x == NULL || v == NULL || id == NULL || type == NULL)
+        return serror_(space_err_null);
+    for (cid = 0; cid < s->nr_cells; cid++) {
+        if (count[cid] < 0) return serror_(space_err_range);
+        n += count[cid];
+    }
+    if (n != s->nr_parts) return serror_(space_err_range);
+    if (posix_memalign((void **)&save, cell_partalign, sizeof(struct part) * (n > 0 ? n : 1)) != 0)
+        return serror_(space_err_malloc);
+    for (a = 0; a < n; a++) {
+        if (id[a] < 0 || id[a] >= s->size_parts || s->partlist[id[a]] == NULL) {
+            free(save);
+            return serror_(space_err_range);
+        }
+        save[a] = *s->partlist[id[a]];
+    }
+    for (a = 0, cid = 0; cid < s->nr_cells; cid++) {
+        struct cell *c = &s->cells[cid];
+        c->count = 0;
+        if (count[cid] > c->size && cell_grow(c, count[cid]) < 0) {
+            free(save);
+            return serror_(space_err_malloc);
+        }
+        for (k = 0; k < count[cid]; k++, a++) {
+            struct part *p = &c->parts[k];
+            *p = save[a];
+            for (m = 0; m < 3; m++) {
+                p->x[m] = x[3 * a + m];
+                p->v[m] = v[3 * a + m];
+            }
+            p->type = (short)type[a];
+            s->partlist[p->id] = p;
+            s->celllist[p->id] = c;
+        }
+        c->count = count[cid];
+    }
+    free(save);
+    return space_err_ok;
+}
+
 void space_free(struct space *s) {
     int cid;
     if (s == NULL) return;

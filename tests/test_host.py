@@ -59,6 +59,9 @@ def test_load_outside_box_and_start_without_gpu_flag():
     with pytest.raises(md.MdcoreError) as ex:
         e.start()
     assert "(-22)" in str(ex.value)  # engine_err_nogpu: no CPU fallback
+    with pytest.raises(md.MdcoreError) as ex:
+        e.gpu_steps(3)                   # device steps need engine_gpu_load
+    assert "(-22)" in str(ex.value)
     e.close()
 
 
