@@ -10,11 +10,15 @@
  * Migration keeps the host's order exactly, so the sorted lists, the pair
  * sets and the trajectory are bitwise those of the host-driven loop:
  *   - per source cell, in storage order, a leaving particle is appended to
- *     its departures and the cell's last particle takes its slot
- *     (k_migrate, one thread per cell, the loop of engine_advance);
- *   - each destination appends its arrivals ordered by source cell and
- *     departure order (k_arrive, one thread per cell), after its stayers;
- *   - new counts and offsets come from one fixed-order scan (k_scan).
+ *     its departures and the cell's last particle takes its slot (k_migrate,
+ *     one wave per cell, with the kick fused in);
+ *   - new counts and offsets come from a fixed-order three-pass scan
+ *     (k_scan_a/b/c), arrival counts from atomic increments (a sum, so the
+ *     order of the increments does not matter);
+ *   - stayers keep their relative order (k_stayers); each destination
+ *     appends its arrivals by source cell and departure order (k_arrive).
+ * The host's shuffle at the start of a step moves nothing when the previous
+ * advance left every particle inside its cell; it is then skipped.
  * No reference counterpart as a device path: the reference's CUDA engine
  * returns to the host every step (runner_cuda.cu:2405-2430, 2514-2521).
  */
@@ -32,184 +36,208 @@ struct step_geom {
     int ncells;
 };
 
-/* v += f dt / m, x += dt v, in the host's FP64 order (engine.c, engine_advance). */
-__global__ __launch_bounds__(256) void k_kick(int n, double dt, const double *__restrict__ imass,
-                                              const int *__restrict__ type, const float4 *__restrict__ fhi,
-                                              const float4 *__restrict__ flo, double *__restrict__ x,
-                                              double *__restrict__ v) {
-#pragma clang fp contract(off)
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
-    const double w = dt * imass[type[a]];
-    const float4 hi = fhi[a], lo = flo[a];
-    /* the host adds the pass's force into a zeroed FP64 force */
-    const double f[3] = {0.0 + ((double)hi.x + (double)lo.x), 0.0 + ((double)hi.y + (double)lo.y),
-                         0.0 + ((double)hi.z + (double)lo.z)};
-    for (int k = 0; k < 3; k++) {
-        v[3 * a + k] += f[k] * w;
-        x[3 * a + k] += dt * v[3 * a + k];
+#define MD_SCAN_CELLS 1024 /* cells per block of the count scan: 256 threads x 4 */
+
+/* Leave code of a particle: 0 stays, else 1 + 9(dx+1) + 3(dy+1) + (dz+1). */
+__device__ __forceinline__ int leave_code(const double *xa, const double *h) {
+    int d[3], any = 0;
+    for (int m = 0; m < 3; m++) {
+        d[m] = (xa[m] >= h[m]) - (xa[m] < 0.0);
+        any |= d[m];
     }
+    return any ? 1 + 9 * (d[0] + 1) + 3 * (d[1] + 1) + (d[2] + 1) : 0;
 }
 
-/* Per source cell: departures in the host's order (space_shuffle_local,
-   engine_advance).  Writes the stayers compacted in place, the departures
-   into the cell's segment of the departure arrays, and slot -> cell of the
-   old layout. */
-__global__ __launch_bounds__(256) void k_migrate(struct step_geom G, const int *__restrict__ count,
-                                                 const int *__restrict__ offset, double *__restrict__ x,
-                                                 double *__restrict__ v, int *__restrict__ id,
-                                                 int *__restrict__ type, int *__restrict__ stay,
-                                                 int *__restrict__ ndep, double *__restrict__ qx,
-                                                 double *__restrict__ qv, int *__restrict__ qid,
-                                                 int *__restrict__ qtype, int *__restrict__ qdest,
-                                                 int *__restrict__ cell_of) {
+/* One wave per cell.  With kick: v += f dt / m, x += dt v in the host's FP64
+   order (engine_advance, engine.c:1945-2052), written back in place, and the
+   cell's kinetic energy.  Then the departures in the host's order: slot pid
+   is examined, a leaving particle is appended to the departures and the
+   cell's last particle takes its slot (space_shuffle_local).  Slots
+   (pid, cnt) always still hold their original particle, so the walk only
+   tracks which original particle sits at pid.  Output per old slot:
+   slot >= 0, the particle's slot among the stayers, or -1 - its departure
+   index; per departure its source slot and destination cell. */
+__global__ __launch_bounds__(64) void k_migrate(struct step_geom G, int kick, double dt,
+                                                const double *__restrict__ mass, const double *__restrict__ imass,
+                                                const float4 *__restrict__ fhi, const float4 *__restrict__ flo,
+                                                const int *__restrict__ count, const int *__restrict__ offset,
+                                                double *__restrict__ x, double *__restrict__ v,
+                                                const int *__restrict__ type, int *__restrict__ cell_of,
+                                                int *__restrict__ slot, int *__restrict__ qsrc,
+                                                int *__restrict__ qdest, int *__restrict__ stay,
+                                                int *__restrict__ ndep, int *__restrict__ narr,
+                                                double *__restrict__ cell_ke) {
 #pragma clang fp contract(off)
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= G.ncells) return;
-    const int off = offset[c], n0 = count[c];
+    __shared__ unsigned char code[MD_MAX_CELL];
+    __shared__ short res[MD_MAX_CELL];
+    __shared__ int sh_stay, sh_nd;
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int off = offset[c], n = count[c];
+    double ke = 0.0;
+    for (int k = lane; k < n; k += 64) {
+        const int a = off + k;
+        double xa[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
+        if (kick) {
+            const int t = type[a];
+            const double w = dt * imass[t];
+            const float4 hi = fhi[a], lo = flo[a];
+            /* the host adds the pass's force into a zeroed FP64 force */
+            const double f[3] = {0.0 + ((double)hi.x + (double)lo.x), 0.0 + ((double)hi.y + (double)lo.y),
+                                 0.0 + ((double)hi.z + (double)lo.z)};
+            double vv[3];
+            for (int m = 0; m < 3; m++) {
+                vv[m] = v[3 * a + m] + f[m] * w;
+                xa[m] = xa[m] + dt * vv[m];
+                v[3 * a + m] = vv[m];
+                x[3 * a + m] = xa[m];
+            }
+            ke += 0.5 * mass[t] * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+        }
+        code[k] = (unsigned char)leave_code(xa, G.h);
+        cell_of[a] = c;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int pid = 0, cnt = n, nd = 0, cur = 0;
+        while (pid < cnt) {
+            if (code[cur]) {
+                res[cur] = (short)(-1 - nd);
+                nd++;
+                cnt--;
+                cur = cnt; /* the last particle, still in its original slot, fills the hole */
+            } else {
+                res[cur] = (short)pid;
+                pid++;
+                cur = pid;
+            }
+        }
+        sh_stay = cnt;
+        sh_nd = nd;
+    }
+    __syncthreads();
     const int loc[3] = {c / (G.cdim[1] * G.cdim[2]), (c / G.cdim[2]) % G.cdim[1], c % G.cdim[2]};
-    for (int k = 0; k < n0; k++) cell_of[off + k] = c;
-    int cnt = n0, nd = 0, pid = 0;
-    while (pid < cnt) {
-        const int a = off + pid;
-        int d[3];
-        for (int k = 0; k < 3; k++) d[k] = (x[3 * a + k] >= G.h[k]) - (x[3 * a + k] < 0.0);
-        if (d[0] || d[1] || d[2]) {
-            const int b = off + nd;
-            for (int k = 0; k < 3; k++) {
-                qx[3 * b + k] = x[3 * a + k] - d[k] * G.h[k];
-                qv[3 * b + k] = v[3 * a + k];
-            }
-            qid[b] = id[a];
-            qtype[b] = type[a];
-            qdest[b] = (((loc[0] + d[0] + G.cdim[0]) % G.cdim[0]) * G.cdim[1] +
-                        (loc[1] + d[1] + G.cdim[1]) % G.cdim[1]) * G.cdim[2] +
-                       (loc[2] + d[2] + G.cdim[2]) % G.cdim[2];
-            nd++;
-            cnt--;
-            if (pid < cnt) {
-                const int e = off + cnt;
-                for (int k = 0; k < 3; k++) {
-                    x[3 * a + k] = x[3 * e + k];
-                    v[3 * a + k] = v[3 * e + k];
-                }
-                id[a] = id[e];
-                type[a] = type[e];
-            }
-        } else {
-            pid++;
+    for (int k = lane; k < n; k += 64) {
+        const int r = res[k];
+        slot[off + k] = r;
+        if (r < 0) {
+            const int q = off + (-1 - r), cd = code[k] - 1;
+            const int d[3] = {cd / 9 - 1, (cd / 3) % 3 - 1, cd % 3 - 1};
+            const int dest = (((loc[0] + d[0] + G.cdim[0]) % G.cdim[0]) * G.cdim[1] +
+                              (loc[1] + d[1] + G.cdim[1]) % G.cdim[1]) * G.cdim[2] +
+                             (loc[2] + d[2] + G.cdim[2]) % G.cdim[2];
+            qsrc[q] = k;
+            qdest[q] = dest;
+            atomicAdd(&narr[dest], 1);
         }
     }
-    stay[c] = cnt;
-    ndep[c] = nd;
-}
-
-/* The up to 27 cells a particle can arrive from (itself included when a
-   dimension has fewer than 3 cells), ascending and without repeats. */
-__device__ __forceinline__ int source_cells(const struct step_geom &G, int c, int *src) {
-    const int loc[3] = {c / (G.cdim[1] * G.cdim[2]), (c / G.cdim[2]) % G.cdim[1], c % G.cdim[2]};
-    int n = 0;
-    for (int i = -1; i <= 1; i++)
-        for (int j = -1; j <= 1; j++)
-            for (int k = -1; k <= 1; k++) {
-                const int s = (((loc[0] + i + G.cdim[0]) % G.cdim[0]) * G.cdim[1] +
-                               (loc[1] + j + G.cdim[1]) % G.cdim[1]) * G.cdim[2] +
-                              (loc[2] + k + G.cdim[2]) % G.cdim[2];
-                int p = n;
-                while (p > 0 && src[p - 1] > s) {
-                    src[p] = src[p - 1];
-                    p--;
-                }
-                if (p > 0 && src[p - 1] == s) {
-                    for (int q = p; q < n; q++) src[q] = src[q + 1];
-                    continue;
-                }
-                src[p] = s;
-                n++;
-            }
-    return n;
-}
-
-/* Per destination cell: arrivals by source cell then departure order.
-   Without out, only counts them. */
-__global__ __launch_bounds__(128) void k_arrive(struct step_geom G, const int *__restrict__ offset,
-                                                const int *__restrict__ ndep, const int *__restrict__ qdest,
-                                                const double *__restrict__ qx, const double *__restrict__ qv,
-                                                const int *__restrict__ qid, const int *__restrict__ qtype,
-                                                int *__restrict__ narr, const int *__restrict__ new_off,
-                                                const int *__restrict__ stay, double *__restrict__ x2,
-                                                double *__restrict__ v2, int *__restrict__ id2,
-                                                int *__restrict__ type2, int write) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= G.ncells) return;
-    int src[27];
-    const int ns = source_cells(G, c, src);
-    int j = 0;
-    for (int q = 0; q < ns; q++) {
-        const int s = src[q], off = offset[s], nd = ndep[s];
-        for (int k = 0; k < nd; k++) {
-            const int b = off + k;
-            if (qdest[b] != c) continue;
-            if (write) {
-                const int a = new_off[c] + stay[c] + j;
-                for (int m = 0; m < 3; m++) {
-                    x2[3 * a + m] = qx[3 * b + m];
-                    v2[3 * a + m] = qv[3 * b + m];
-                }
-                id2[a] = qid[b];
-                type2[a] = qtype[b];
-            }
-            j++;
-        }
+    if (kick) {
+        for (int s = 32; s > 0; s >>= 1) ke += __shfl_xor(ke, s, 64);
+        if (lane == 0) cell_ke[c] = ke;
     }
-    if (!write) narr[c] = j;
+    if (lane == 0) {
+        stay[c] = sh_stay;
+        ndep[c] = sh_nd;
+    }
 }
 
-/* New counts and offsets: one block, fixed-order exclusive scan; the
-   largest count goes to *maxc. */
-__global__ __launch_bounds__(1024) void k_scan(int ncells, const int *__restrict__ stay,
-                                               const int *__restrict__ narr, int *__restrict__ count,
-                                               int *__restrict__ offset, int2 *__restrict__ cnt_off,
-                                               int *__restrict__ maxc) {
-    __shared__ int part[1024];
-    __shared__ int pmax[1024];
-    const int t = threadIdx.x;
-    const int per = (ncells + 1023) / 1024;
-    const int c0 = t * per, c1 = c0 + per < ncells ? c0 + per : ncells;
+/* Count scan, pass 1: per block of MD_SCAN_CELLS cells the sum and maximum
+   of the new counts (stayers + arrivals) and the sum of the cell energies. */
+__global__ __launch_bounds__(256) void k_scan_a(int ncells, const int *__restrict__ stay,
+                                                const int *__restrict__ narr, const double *__restrict__ cell_ke,
+                                                int *__restrict__ bsum, int *__restrict__ bmax,
+                                                double *__restrict__ bke) {
+    __shared__ int ss[256], sm[256];
+    __shared__ double sk[256];
+    const int t = threadIdx.x, c0 = blockIdx.x * MD_SCAN_CELLS + 4 * t;
     int sum = 0, mx = 0;
-    for (int c = c0; c < c1; c++) {
+    double ke = 0.0;
+    for (int c = c0; c < c0 + 4 && c < ncells; c++) {
         const int n = stay[c] + narr[c];
         sum += n;
         mx = n > mx ? n : mx;
+        if (cell_ke) ke += cell_ke[c];
     }
-    part[t] = sum;
-    pmax[t] = mx;
+    ss[t] = sum;
+    sm[t] = mx;
+    sk[t] = ke;
     __syncthreads();
-    if (t == 0) {
-        int run = 0, m = 0;
-        for (int k = 0; k < 1024; k++) {
-            const int v = part[k];
-            part[k] = run;
-            run += v;
-            m = pmax[k] > m ? pmax[k] : m;
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            ss[t] += ss[t + w];
+            sm[t] = sm[t] > sm[t + w] ? sm[t] : sm[t + w];
+            sk[t] += sk[t + w];
         }
-        offset[ncells] = run;
-        *maxc = m;
+        __syncthreads();
     }
-    __syncthreads();
-    int run = part[t];
-    for (int c = c0; c < c1; c++) {
-        const int n = stay[c] + narr[c];
-        count[c] = n;
-        offset[c] = run;
-        cnt_off[c] = make_int2(n, run);
-        run += n;
+    if (t == 0) {
+        bsum[blockIdx.x] = ss[0];
+        bmax[blockIdx.x] = sm[0];
+        bke[blockIdx.x] = sk[0];
     }
 }
 
-/* Stayers: old slot a of cell c, if still in the cell, to its new slot. */
-__global__ __launch_bounds__(256) void k_stayers(int n, const int *__restrict__ cell_of,
-                                                 const int *__restrict__ old_off, const int *__restrict__ stay,
+/* Pass 2, one thread: block bases, the total, the largest count, the
+   kinetic energy and the pass totals of this step; clears the
+   outside-the-cell flag. */
+__global__ __launch_bounds__(64) void k_scan_b(int nblocks, int ncells, int *__restrict__ bsum,
+                                               const int *__restrict__ bmax, const double *__restrict__ bke,
+                                               int *__restrict__ offset, int *__restrict__ st,
+                                               const double *__restrict__ tot, double *__restrict__ esteps) {
+    if (threadIdx.x != 0) return;
+    int run = 0, mx = 0;
+    double ke = 0.0;
+    for (int b = 0; b < nblocks; b++) {
+        const int s = bsum[b];
+        bsum[b] = run;
+        run += s;
+        mx = bmax[b] > mx ? bmax[b] : mx;
+        ke += bke[b];
+    }
+    offset[ncells] = run;
+    st[0] = mx;
+    st[1] = 0;
+    if (esteps) {
+        esteps[0] = tot[0];
+        esteps[1] = tot[1];
+        esteps[2] = tot[2];
+        esteps[3] = ke;
+    }
+}
+
+/* Pass 3: counts, offsets and (count, offset) records of the new layout;
+   clears the arrival counts for the next migration. */
+__global__ __launch_bounds__(256) void k_scan_c(int ncells, const int *__restrict__ stay, int *__restrict__ narr,
+                                                const int *__restrict__ bbase, int *__restrict__ count,
+                                                int *__restrict__ offset, int2 *__restrict__ cnt_off) {
+    __shared__ int ss[256];
+    const int t = threadIdx.x, c0 = blockIdx.x * MD_SCAN_CELLS + 4 * t;
+    int n[4] = {0, 0, 0, 0}, sum = 0;
+    for (int k = 0; k < 4; k++)
+        if (c0 + k < ncells) {
+            n[k] = stay[c0 + k] + narr[c0 + k];
+            sum += n[k];
+        }
+    ss[t] = sum;
+    __syncthreads();
+    for (int w = 1; w < 256; w <<= 1) {
+        const int add = t >= w ? ss[t - w] : 0;
+        __syncthreads();
+        ss[t] += add;
+        __syncthreads();
+    }
+    int run = bbase[blockIdx.x] + ss[t] - sum;
+    for (int k = 0; k < 4; k++)
+        if (c0 + k < ncells) {
+            count[c0 + k] = n[k];
+            offset[c0 + k] = run;
+            cnt_off[c0 + k] = make_int2(n[k], run);
+            narr[c0 + k] = 0;
+            run += n[k];
+        }
+}
+
+/* Stayers: old slot a to its slot in the new layout. */
+__global__ __launch_bounds__(256) void k_stayers(int n, const int *__restrict__ cell_of, const int *__restrict__ slot,
                                                  const int *__restrict__ new_off, const double *__restrict__ x,
                                                  const double *__restrict__ v, const int *__restrict__ id,
                                                  const int *__restrict__ type, double *__restrict__ x2,
@@ -217,9 +245,9 @@ __global__ __launch_bounds__(256) void k_stayers(int n, const int *__restrict__ 
                                                  int *__restrict__ type2) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
-    const int c = cell_of[a], k = a - old_off[c];
-    if (k >= stay[c]) return;
-    const int b = new_off[c] + k;
+    const int r = slot[a];
+    if (r < 0) return;
+    const int b = new_off[cell_of[a]] + r;
     for (int m = 0; m < 3; m++) {
         x2[3 * b + m] = x[3 * a + m];
         v2[3 * b + m] = v[3 * a + m];
@@ -228,46 +256,83 @@ __global__ __launch_bounds__(256) void k_stayers(int n, const int *__restrict__ 
     type2[b] = type[a];
 }
 
-/* Kinetic energy 1/2 m v^2 per cell in storage order, then in cell order. */
-__global__ __launch_bounds__(256) void k_kin_cells(int ncells, const int *__restrict__ count,
-                                                   const int *__restrict__ offset, const double *__restrict__ mass,
-                                                   const int *__restrict__ type, const double *__restrict__ v,
-                                                   double *__restrict__ cell_ke) {
-#pragma clang fp contract(off)
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    double s = 0.0;
-    for (int k = offset[c]; k < offset[c] + count[c]; k++) {
-        const double m = mass[type[k]];
-        s += 0.5 * m * (v[3 * k] * v[3 * k] + v[3 * k + 1] * v[3 * k + 1] + v[3 * k + 2] * v[3 * k + 2]);
-    }
-    cell_ke[c] = s;
+/* The wrapped coordinates l-1, l, l+1 of one axis in ascending order; a
+   value equal to its predecessor is a repeat (fewer than 3 cells). */
+__device__ __forceinline__ void sorted3(int l, int cdim, int *s) {
+    int a = (l - 1 + cdim) % cdim, b = l, c = (l + 1) % cdim, t;
+    if (a > b) t = a, a = b, b = t;
+    if (b > c) t = b, b = c, c = t;
+    if (a > b) t = a, a = b, b = t;
+    s[0] = a;
+    s[1] = b;
+    s[2] = c;
 }
 
-__global__ __launch_bounds__(1024) void k_sum1(int n, const double *__restrict__ a, double *__restrict__ out) {
-    __shared__ double sd[1024];
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int k = t; k < n; k += 1024) s += a[k];
-    sd[t] = s;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) sd[t] += sd[t + w];
-        __syncthreads();
+/* Arrivals, one thread per destination cell: appended after the stayers in
+   the host's order, by source cell id and then departure order (the source
+   cells' ids ascend with their wrapped coordinates taken in lexicographic
+   order).  Sets st[1] when an arrival is still outside its new cell. */
+__global__ __launch_bounds__(128) void k_arrive(struct step_geom G, const int *__restrict__ old_off,
+                                                const int *__restrict__ ndep, const int *__restrict__ qsrc,
+                                                const int *__restrict__ qdest, const int *__restrict__ new_off,
+                                                const int *__restrict__ stay, const double *__restrict__ x,
+                                                const double *__restrict__ v, const int *__restrict__ id,
+                                                const int *__restrict__ type, double *__restrict__ x2,
+                                                double *__restrict__ v2, int *__restrict__ id2,
+                                                int *__restrict__ type2, int *__restrict__ st) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G.ncells) return;
+    const int loc[3] = {c / (G.cdim[1] * G.cdim[2]), (c / G.cdim[2]) % G.cdim[1], c % G.cdim[2]};
+    int I[3], J[3], K[3];
+    sorted3(loc[0], G.cdim[0], I);
+    sorted3(loc[1], G.cdim[1], J);
+    sorted3(loc[2], G.cdim[2], K);
+    int b = new_off[c] + stay[c], outside = 0;
+#pragma unroll
+    for (int qi = 0; qi < 3; qi++) {
+        if (qi > 0 && I[qi] == I[qi - 1]) continue;
+#pragma unroll
+        for (int qj = 0; qj < 3; qj++) {
+            if (qj > 0 && J[qj] == J[qj - 1]) continue;
+#pragma unroll
+            for (int qk = 0; qk < 3; qk++) {
+                if (qk > 0 && K[qk] == K[qk - 1]) continue;
+                const int s = (I[qi] * G.cdim[1] + J[qj]) * G.cdim[2] + K[qk];
+                const int off = old_off[s], nd = ndep[s];
+                for (int q = off; q < off + nd; q++) {
+                    if (qdest[q] != c) continue;
+                    const int a = off + qsrc[q];
+                    for (int m = 0; m < 3; m++) {
+                        const double xm = x[3 * a + m];
+                        const int d = (xm >= G.h[m]) - (xm < 0.0);
+                        const double xn = xm - d * G.h[m];
+                        x2[3 * b + m] = xn;
+                        v2[3 * b + m] = v[3 * a + m];
+                        outside |= (xn >= G.h[m]) | (xn < 0.0);
+                    }
+                    id2[b] = id[a];
+                    type2[b] = type[a];
+                    b++;
+                }
+            }
+        }
     }
-    if (t == 0) *out = sd[0];
+    if (outside) st[1] = 1;
 }
 
 /* ------------------------------------------------------------ host side */
 
 struct step_bufs {
-    double *v, *x2, *v2, *qx, *qv, *mass, *imass, *cell_ke, *esteps;
-    int *id2, *type2, *qid, *qtype, *qdest, *stay, *ndep, *narr, *maxc, *count2, *offset2;
+    double *v, *x2, *v2, *mass, *imass, *cell_ke, *bke, *esteps;
+    int *id2, *type2, *slot, *qsrc, *qdest, *stay, *ndep, *narr, *bsum, *bmax, *st, *count2, *offset2;
+    int nblocks;
 };
 
 static void step_free(struct step_bufs *b) {
-    double *d[] = {b->v, b->x2, b->v2, b->qx, b->qv, b->mass, b->imass, b->cell_ke, b->esteps};
-    int *i[] = {b->id2, b->type2, b->qid, b->qtype, b->qdest, b->stay, b->ndep, b->narr, b->maxc, b->count2, b->offset2};
+    double *d[] = {b->v, b->x2, b->v2, b->mass, b->imass, b->cell_ke, b->bke, b->esteps};
+    int *i[] = {b->id2, b->type2, b->slot, b->qsrc, b->qdest, b->stay, b->ndep,
+                b->narr, b->bsum, b->bmax, b->st, b->count2, b->offset2};
     for (size_t k = 0; k < sizeof(d) / sizeof(d[0]); k++) (void)hipFree(d[k]);
     for (size_t k = 0; k < sizeof(i) / sizeof(i[0]); k++) (void)hipFree(i[k]);
     memset(b, 0, sizeof(*b));
@@ -276,43 +341,51 @@ static void step_free(struct step_bufs *b) {
 static int step_alloc(struct step_bufs *b, size_t N, int ncells, int ntypes, int nsteps) {
     memset(b, 0, sizeof(*b));
     const size_t C = (size_t)ncells;
-    if (hipMalloc(&b->v, sizeof(double) * 3 * N) != hipSuccess || hipMalloc(&b->x2, sizeof(double) * 3 * N) != hipSuccess ||
-        hipMalloc(&b->v2, sizeof(double) * 3 * N) != hipSuccess || hipMalloc(&b->qx, sizeof(double) * 3 * N) != hipSuccess ||
-        hipMalloc(&b->qv, sizeof(double) * 3 * N) != hipSuccess || hipMalloc(&b->mass, sizeof(double) * ntypes) != hipSuccess ||
-        hipMalloc(&b->imass, sizeof(double) * ntypes) != hipSuccess || hipMalloc(&b->cell_ke, sizeof(double) * C) != hipSuccess ||
+    b->nblocks = (ncells + MD_SCAN_CELLS - 1) / MD_SCAN_CELLS;
+    const size_t B = (size_t)b->nblocks;
+    if (hipMalloc(&b->v, sizeof(double) * 3 * N) != hipSuccess ||
+        hipMalloc(&b->x2, sizeof(double) * 3 * N) != hipSuccess ||
+        hipMalloc(&b->v2, sizeof(double) * 3 * N) != hipSuccess ||
+        hipMalloc(&b->mass, sizeof(double) * ntypes) != hipSuccess ||
+        hipMalloc(&b->imass, sizeof(double) * ntypes) != hipSuccess ||
+        hipMalloc(&b->cell_ke, sizeof(double) * C) != hipSuccess || hipMalloc(&b->bke, sizeof(double) * B) != hipSuccess ||
         hipMalloc(&b->esteps, sizeof(double) * 4 * (size_t)nsteps) != hipSuccess ||
         hipMalloc(&b->id2, sizeof(int) * N) != hipSuccess || hipMalloc(&b->type2, sizeof(int) * N) != hipSuccess ||
-        hipMalloc(&b->qid, sizeof(int) * N) != hipSuccess || hipMalloc(&b->qtype, sizeof(int) * N) != hipSuccess ||
+        hipMalloc(&b->slot, sizeof(int) * N) != hipSuccess || hipMalloc(&b->qsrc, sizeof(int) * N) != hipSuccess ||
         hipMalloc(&b->qdest, sizeof(int) * N) != hipSuccess || hipMalloc(&b->stay, sizeof(int) * C) != hipSuccess ||
         hipMalloc(&b->ndep, sizeof(int) * C) != hipSuccess || hipMalloc(&b->narr, sizeof(int) * C) != hipSuccess ||
-        hipMalloc(&b->maxc, sizeof(int)) != hipSuccess || hipMalloc(&b->count2, sizeof(int) * C) != hipSuccess ||
-        hipMalloc(&b->offset2, sizeof(int) * (C + 1)) != hipSuccess) {
+        hipMalloc(&b->bsum, sizeof(int) * B) != hipSuccess || hipMalloc(&b->bmax, sizeof(int) * B) != hipSuccess ||
+        hipMalloc(&b->st, sizeof(int) * 2) != hipSuccess || hipMalloc(&b->count2, sizeof(int) * C) != hipSuccess ||
+        hipMalloc(&b->offset2, sizeof(int) * (C + 1)) != hipSuccess ||
+        hipMemset(b->narr, 0, sizeof(int) * C) != hipSuccess) {
         step_free(b);
         return -1;
     }
     return 0;
 }
 
-/* Shuffle (kick == 0) or advance: kick-drift, migrate, rebuild the cell
-   layout and swap it in.  Asynchronous; the largest new cell count lands in
-   b->maxc. */
-static int step_migrate(struct gpu_state *g, struct step_bufs *b, const struct step_geom &G, int kick, double dt) {
-    const int n = g->nparts, nc = g->ncells;
-    const int nbp = n > 0 ? (n + 255) / 256 : 1, nbc = (nc + 255) / 256, nba = (nc + 127) / 128;
-    if (kick)
-        hipLaunchKernelGGL(k_kick, dim3(nbp), dim3(256), 0, g->stream, n, dt, b->imass, g->d_type, g->d_fhi,
-                           g->d_flo, g->d_x, b->v);
-    hipLaunchKernelGGL(k_migrate, dim3(nbc), dim3(256), 0, g->stream, G, g->d_count, g->d_offset, g->d_x, b->v,
-                       g->d_id, g->d_type, b->stay, b->ndep, b->qx, b->qv, b->qid, b->qtype, b->qdest, g->d_cell_of);
-    hipLaunchKernelGGL(k_arrive, dim3(nba), dim3(128), 0, g->stream, G, g->d_offset, b->ndep, b->qdest, b->qx, b->qv,
-                       b->qid, b->qtype, b->narr, (const int *)NULL, (const int *)NULL, (double *)NULL, (double *)NULL,
-                       (int *)NULL, (int *)NULL, 0);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, g->stream, nc, b->stay, b->narr, b->count2, b->offset2,
-                       g->d_cnt_off, b->maxc);
-    hipLaunchKernelGGL(k_stayers, dim3(nbp), dim3(256), 0, g->stream, n, g->d_cell_of, g->d_offset, b->stay,
-                       b->offset2, g->d_x, b->v, g->d_id, g->d_type, b->x2, b->v2, b->id2, b->type2);
-    hipLaunchKernelGGL(k_arrive, dim3(nba), dim3(128), 0, g->stream, G, g->d_offset, b->ndep, b->qdest, b->qx, b->qv,
-                       b->qid, b->qtype, b->narr, b->offset2, b->stay, b->x2, b->v2, b->id2, b->type2, 1);
+/* Shuffle (es == NULL: no kick) or advance (kick-drift, then migration, the
+   step's energies into es[0..3]): departures, the new layout, stayers and
+   arrivals copied into it, then the layouts swapped.  Asynchronous; the
+   largest new cell count and the outside flag land in b->st. */
+static int step_migrate(struct gpu_state *g, struct step_bufs *b, const struct step_geom &G, double dt,
+                        double *es) {
+    const int n = g->nparts, nc = g->ncells, kick = es != NULL;
+    const int nbp = n > 0 ? (n + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_migrate, dim3(nc), dim3(64), 0, g->stream, G, kick, dt, b->mass, b->imass, g->d_fhi,
+                       g->d_flo, g->d_count, g->d_offset, g->d_x, b->v, g->d_type, g->d_cell_of, b->slot, b->qsrc,
+                       b->qdest, b->stay, b->ndep, b->narr, b->cell_ke);
+    hipLaunchKernelGGL(k_scan_a, dim3(b->nblocks), dim3(256), 0, g->stream, nc, b->stay, b->narr,
+                       kick ? b->cell_ke : (const double *)NULL, b->bsum, b->bmax, b->bke);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(64), 0, g->stream, b->nblocks, nc, b->bsum, b->bmax, b->bke,
+                       b->offset2, b->st, g->d_tot, es);
+    hipLaunchKernelGGL(k_scan_c, dim3(b->nblocks), dim3(256), 0, g->stream, nc, b->stay, b->narr, b->bsum,
+                       b->count2, b->offset2, g->d_cnt_off);
+    hipLaunchKernelGGL(k_stayers, dim3(nbp), dim3(256), 0, g->stream, n, g->d_cell_of, b->slot, b->offset2, g->d_x,
+                       b->v, g->d_id, g->d_type, b->x2, b->v2, b->id2, b->type2);
+    hipLaunchKernelGGL(k_arrive, dim3((nc + 127) / 128), dim3(128), 0, g->stream, G, g->d_offset, b->ndep, b->qsrc,
+                       b->qdest, b->offset2, b->stay, g->d_x, b->v, g->d_id, g->d_type, b->x2, b->v2, b->id2,
+                       b->type2, b->st);
     HIPCHECK(hipGetLastError());
     double *t;
     int *u;
@@ -322,6 +395,19 @@ static int step_migrate(struct gpu_state *g, struct step_bufs *b, const struct s
     u = g->d_type, g->d_type = b->type2, b->type2 = u;
     u = g->d_count, g->d_count = b->count2, b->count2 = u;
     u = g->d_offset, g->d_offset = b->offset2, b->offset2 = u;
+    return engine_err_ok;
+}
+
+/* Wait for the last migration; its largest cell count and whether an
+   arrival is still outside its cell. */
+static int step_status(struct gpu_state *g, struct step_bufs *b, int *maxc, int *outside) {
+    int st[2];
+    if (hipMemcpyAsync(st, b->st, sizeof(st), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess)
+        return gerror_(engine_err_gpu, "migration failed");
+    *maxc = st[0];
+    *outside = st[1];
+    if (st[0] > MD_MAX_CELL) return gerror_(engine_err_maxparts, "more than 1024 particles in a cell");
     return engine_err_ok;
 }
 
@@ -401,7 +487,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     struct space *s;
     struct step_bufs b;
     struct step_geom G;
-    int rc = engine_err_ok, maxc = 0, flag = 0;
+    int rc = engine_err_ok, maxc = 0, outside = 0, flag = 0;
     if (e == NULL) return gerror_(engine_err_null, "NULL engine");
     g = (struct gpu_state *)e->gpu;
     if (g == NULL) return gerror_(engine_err_nogpu, "engine_gpu_load was not called");
@@ -441,34 +527,20 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
         free(imass);
         free(v);
     }
+    /* engine_step: shuffle, pass, advance.  After an advance that left every
+       particle inside its cell, the next shuffle moves nothing and is skipped. */
+    outside = 1;
     for (int step = 0; step < nsteps && rc == engine_err_ok; step++) {
-        /* engine_step: shuffle, then the pass on the new layout */
-        if (step_migrate(g, &b, G, 0, 0.0) < 0 ||
-            hipMemcpyAsync(&maxc, b.maxc, sizeof(int), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
-            hipStreamSynchronize(g->stream) != hipSuccess) {
-            rc = gerror_(engine_err_gpu, "shuffle failed");
-            break;
-        }
-        if (maxc > MD_MAX_CELL) {
-            rc = gerror_(engine_err_maxparts, "more than 1024 particles in a cell");
-            break;
+        if (outside) {
+            if ((rc = step_migrate(g, &b, G, 0.0, NULL)) < 0 || (rc = step_status(g, &b, &maxc, &outside)) < 0)
+                break;
         }
         g->maxcount = maxc;
         g->P.maxn = maxc > 0 ? maxc : 1;
-        if (gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0) < 0) {
-            rc = engine_err_gpu;
+        if ((rc = gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0)) < 0) break;
+        if ((rc = step_migrate(g, &b, G, e->dt, b.esteps + 4 * step)) < 0 ||
+            (rc = step_status(g, &b, &maxc, &outside)) < 0)
             break;
-        }
-        /* the pass's totals, then kick-drift and migration (engine_advance) */
-        if (hipMemcpyAsync(b.esteps + 4 * step, g->d_tot, sizeof(double) * 3, hipMemcpyDeviceToDevice, g->stream) !=
-                hipSuccess ||
-            step_migrate(g, &b, G, 1, e->dt) < 0) {
-            rc = gerror_(engine_err_gpu, "advance failed");
-            break;
-        }
-        hipLaunchKernelGGL(k_kin_cells, dim3((G.ncells + 255) / 256), dim3(256), 0, g->stream, G.ncells, g->d_count,
-                           g->d_offset, b.mass, g->d_type, b.v, b.cell_ke);
-        hipLaunchKernelGGL(k_sum1, dim3(1), dim3(1024), 0, g->stream, G.ncells, b.cell_ke, b.esteps + 4 * step + 3);
         e->time += 1;
     }
     if (rc == engine_err_ok &&
