@@ -21,6 +21,10 @@ passes timed with HIP events on the engine's stream.  Prints one JSON line.
                 whose force error matches the network's, timed the same way
   nve           energy drift of 1,000 NVE steps (dt 2 fs) of the 32,000-atom
                 argon system with the table pass and with a trained network
+  md_step       whole MD steps (shuffle, pass, kick-drift, migration) of the
+                1,048,576-atom argon crystal at 100 K, dt 2 fs: host-driven
+                engine_step (positions up, forces down, host integration)
+                against engine_gpu_steps (particles resident on the device)
 """
 import argparse
 import json
@@ -102,6 +106,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate section")
     ap.add_argument("--no-nve", action="store_true", help="skip the NVE drift section")
+    ap.add_argument("--md-steps", type=int, default=100,
+                    help="resident MD steps timed in the md_step section (0 skips it)")
+    ap.add_argument("--md-only", action="store_true", help="run only the md_step section")
     ap.add_argument("--engine-steps", type=int, default=3, help="full engine_step calls timed for ns/day")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times, as the "
@@ -113,6 +120,10 @@ def main():
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+
+    if args.md_only:
+        print(json.dumps({"md_step": md_step_section(args.md_steps)}), flush=True)
         return
 
     import mdcore as md
@@ -198,6 +209,8 @@ def main():
     if not args.no_nve:
         out["nve"] = nve_section()
     eng.close()
+    if args.md_steps > 0:
+        out["md_step"] = md_step_section(args.md_steps)
     print(json.dumps(out), flush=True)
 
 
@@ -326,6 +339,43 @@ def nve_section(steps=1000, dt=0.002, equil=1000):
                       "e_std": float(e_tab.std()), "wall_s": round(t1 - t0, 1)},
             "mlp": {"drift_rel_per_ns": rm, "drift_kJ_mol_per_ns": am, "e_mean": float(e_mlp.mean()),
                     "e_std": float(e_mlp.std()), "wall_s": round(t2 - t1, 1)}}
+
+
+def md_step_section(nsteps, host_steps=5, dt=0.002):
+    """ms per whole MD step of the 1,048,576-atom argon crystal (fcc 64^3,
+    100 K): host-driven engine_step against engine_gpu_steps.  The resident
+    time is the wall time of one engine_gpu_steps call over nsteps steps,
+    including its one upload and one download."""
+    import numpy as np
+    import mdcore as md
+    import systems
+    sysm = systems.argon(64, T=100.0)
+    eng = systems.build_engine(sysm, md.engine_flag_gpu)
+    eng.setdt(dt)
+    eng.step()
+    t0 = time.perf_counter()
+    for _ in range(host_steps):
+        eng.step()
+    host_ms = (time.perf_counter() - t0) / host_steps * 1e3
+    eng.gpu_steps(3)
+    t0 = time.perf_counter()
+    eng.gpu_steps(10)
+    t_short = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    eng.gpu_steps(nsteps)
+    t_long = time.perf_counter() - t0
+    eng.close()
+    res_ms = t_long / nsteps * 1e3
+    # two call lengths separate the per-step cost from the per-call upload/download
+    step_ms = (t_long - t_short) / (nsteps - 10) * 1e3 if nsteps > 10 else res_ms
+    call_ms = t_short * 1e3 - 10 * step_ms
+    return {"system": "argon_1m: 1,048,576 LJ atoms, fcc 64^3 at 100 K, rc 1.0 nm, 1.0 nm cells",
+            "dt_ps": dt, "steps": nsteps,
+            "host_driven_ms_per_step": round(host_ms, 3), "resident_ms_per_step": round(res_ms, 3),
+            "resident_ms_per_step_marginal": round(step_ms, 3), "resident_ms_per_call": round(call_ms, 1),
+            "ns_per_day_host_driven": dt * 1e-3 * 86400.0 / (host_ms * 1e-3),
+            "ns_per_day_resident": dt * 1e-3 * 86400.0 / (res_ms * 1e-3),
+            "ns_per_day_resident_marginal": dt * 1e-3 * 86400.0 / (step_ms * 1e-3)}
 
 
 def copy_calibration(mib):
