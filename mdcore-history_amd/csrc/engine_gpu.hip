@@ -4,11 +4,14 @@
  * One nonbonded pass is four launches on one stream:
  *   k_pack      cell-ordered FP64 positions -> float4 (x, y, z, type bits),
  *               id -> slot map, slot -> cell map;
- *   k_sort      runner_dosort (runner_dosort.c:97-161) for every cell and
+ *   k_sort_rank runner_dosort (runner_dosort.c:97-161) for every cell and
  *               every flagged direction: 16-bit projection keys computed in
- *               FP64 without contraction, then the reference's descending
- *               quicksort (runner.c:110-174) replayed so that the sorted
- *               lists, ties included, are the reference's;
+ *               FP64 without contraction; particles with a key of their own
+ *               are placed by rank, and a list with equal keys is sorted by
+ *               the reference's descending quicksort (runner.c:110-174)
+ *               replayed, so that the sorted lists, ties included, are the
+ *               reference's (k_sort: the same per list in global memory, for
+ *               cells of more than 191 particles);
  *   k_nonbond   persistent kernel: each wave pulls an owner cell from one of
  *               8 device queues (one per XCD slab of cells), walks the cell's
  *               neighbour list (the reference's self/pair tasks that touch
@@ -301,36 +304,67 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
     sort_desc(list, n, flag);
 }
 
-/* The same sort with each thread's segment and quicksort stack in a private
-   slice of LDS (stride words per thread, odd so that equal indices of
-   different threads fall in different banks): the global sort list is
-   written once, after sorting, instead of being swapped in place.  For
-   cells of at most MD_SORT_LDS_MAX particles. */
-#define MD_SORT_LDS_MAX 191 /* 64 threads x (191 + 32) words = 57 KB of LDS */
-__global__ __launch_bounds__(64) void k_sort_lds(struct nb_params P, const int *__restrict__ count,
-                                                 const int *__restrict__ offset,
-                                                 const unsigned int *__restrict__ sortflags,
-                                                 const double *__restrict__ x, unsigned int *__restrict__ sortlist,
-                                                 unsigned short *__restrict__ keyof, int *__restrict__ flag,
-                                                 int stride) {
-    extern __shared__ unsigned int sh_sort[];
-    unsigned int *list = sh_sort + threadIdx.x * stride;
-    int *stk = (int *)(sh_sort + 64 * stride) + threadIdx.x * 2 * MD_SORT_STACK;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int cid = t / 13, sid = t - 13 * (t / 13);
-    if (cid >= P.ncells) return;
-    if (!((sortflags[cid] >> sid) & 1u)) return;
-    int n = count[cid];
+/* Sort by rank: one wave per cell, all 13 directions at once.  A key that no
+   other particle of its list shares has exactly (number of greater keys)
+   particles before it in any descending order, the reference's quicksort
+   included, so those particles are placed by counting.  Only the order of
+   equal keys depends on the quicksort's partitions: a list with equal keys
+   is sorted again by one lane with the reference's algorithm
+   (runner.c:110-174), in LDS, and written whole.  For cells of at most
+   MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions, 13 * maxn list
+   words, 13 quicksort stacks. */
+#define MD_SORT_LDS_MAX 191
+__global__ __launch_bounds__(64) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
+                                                  const int *__restrict__ offset,
+                                                  const unsigned int *__restrict__ sortflags,
+                                                  const double *__restrict__ x, unsigned int *__restrict__ sortlist,
+                                                  unsigned short *__restrict__ keyof, int *__restrict__ flag,
+                                                  int maxn) {
+    extern __shared__ double sh_rank[];
+    double *xs = sh_rank;
+    unsigned int *lists = (unsigned int *)(sh_rank + 3 * maxn);
+    int *stk = (int *)(lists + 13 * maxn);
+    __shared__ int tie[13];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int n = count[c];
     if (n == 0) return;
-    int off = offset[cid];
-    for (int i = 0; i < n; i++) {
-        unsigned int key = sort_key(&x[3 * (size_t)(off + i)], sid, P.dscale, P.bias);
-        list[i] = ((unsigned int)i << 16) | key;
+    const unsigned int fl = sortflags[c];
+    const int off = offset[c];
+    if (lane < 13) tie[lane] = 0;
+    for (int k = lane; k < 3 * n; k += 64) xs[k] = x[3 * (size_t)off + k];
+    __syncthreads();
+    for (int t = lane; t < 13 * n; t += 64) {
+        const int sid = t / n, i = t - sid * n;
+        if (!((fl >> sid) & 1u)) continue;
+        const unsigned int key = sort_key(&xs[3 * i], sid, P.dscale, P.bias);
+        lists[sid * n + i] = ((unsigned int)i << 16) | key;
         keyof[(size_t)sid * P.nparts + off + i] = (unsigned short)key;
     }
-    sort_desc_t(list, n, stk, stk + MD_SORT_STACK, flag);
-    unsigned int *out = sortlist + 13 * (size_t)off + (size_t)sid * n;
-    for (int i = 0; i < n; i++) out[i] = list[i];
+    __syncthreads();
+    unsigned int *out = sortlist + 13 * (size_t)off;
+    for (int t = lane; t < 13 * n; t += 64) {
+        const int sid = t / n, i = t - sid * n;
+        if (!((fl >> sid) & 1u)) continue;
+        const unsigned int *L = lists + sid * n;
+        const unsigned int w = L[i], key = w & 0xffffu;
+        int gt = 0, eq = 0;
+        for (int j = 0; j < n; j++) {
+            const unsigned int kj = L[j] & 0xffffu;
+            gt += kj > key;
+            eq += kj == key;
+        }
+        if (eq == 1)
+            out[sid * n + gt] = w;
+        else
+            tie[sid] = 1;
+    }
+    __syncthreads();
+    if (lane < 13 && ((fl >> lane) & 1u) && tie[lane]) {
+        unsigned int *L = lists + lane * n;
+        int *st = stk + lane * 2 * MD_SORT_STACK;
+        sort_desc_t(L, n, st, st + MD_SORT_STACK, flag);
+        for (int i = 0; i < n; i++) out[lane * n + i] = L[i];
+    }
 }
 
 /* ------------------------------------------------------------ k_nonbond */
@@ -1515,11 +1549,11 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     hipLaunchKernelGGL(k_pack, dim3(nb), dim3(256), 0, g->stream, g->nparts, g->d_x, g->d_type, g->d_id, g->ncells,
                        g->d_offset, g->d_xf, g->d_index_of_id, g->d_cell_of, g->d_queue);
     if (g->maxcount <= MD_SORT_LDS_MAX) {
-        const int stride = g->maxcount | 1;
-        const size_t sh = sizeof(unsigned int) * 64 * ((size_t)stride + 2 * MD_SORT_STACK);
-        nb = (g->ncells * 13 + 63) / 64;
-        hipLaunchKernelGGL(k_sort_lds, dim3(nb), dim3(64), sh, g->stream, P, g->d_count, g->d_offset,
-                           g->d_sortflags, g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE, stride);
+        const int maxn = g->maxcount > 0 ? g->maxcount : 1;
+        const size_t sh = sizeof(double) * 3 * maxn + sizeof(unsigned int) * 13 * maxn +
+                          sizeof(int) * 13 * 2 * MD_SORT_STACK;
+        hipLaunchKernelGGL(k_sort_rank, dim3(g->ncells), dim3(64), sh, g->stream, P, g->d_count, g->d_offset,
+                           g->d_sortflags, g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE, maxn);
     } else {
         nb = (g->ncells * 13 + 255) / 256;
         hipLaunchKernelGGL(k_sort, dim3(nb), dim3(256), 0, g->stream, P, g->d_count, g->d_offset, g->d_sortflags,
