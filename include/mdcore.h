@@ -186,6 +186,10 @@ struct cell {
     struct part *incomming;
     int incomming_size, incomming_count;
     FPTYPE maxdx;
+    /* Positions at the last pseudo-Verlet rebuild, 3 per particle slot
+       (the reference's c->oldx, 4 per slot, engine.c:286-297). */
+    FPTYPE *oldx;
+    int oldx_size;
     /* 13 sorted lists of (index << 16 | key), count entries each (cell.c:443). */
     unsigned int *sortlist;
     /* Bitmask of the sort directions this cell's pair tasks need (space.c:745-752). */
@@ -352,6 +356,10 @@ struct exclusion {
    with more than one non-zero component and span > 1. */
 #define engine_flag_projwindow    65536
 
+/* Pseudo-Verlet rebuild threshold: maxdx > cutoff * engine_maxdxratio
+   (engine.h:83). */
+#define engine_maxdxratio 0.25
+
 extern int engine_err;
 extern const char *engine_err_msg[];
 
@@ -397,6 +405,13 @@ int engine_start(struct engine *e, int nr_runners, int nr_queues);
 int engine_step(struct engine *e);
 int engine_nonbond_eval(struct engine *e);
 int engine_advance(struct engine *e);
+/* Pseudo-Verlet bookkeeping at the start of a step (engine.c:193-315). */
+int engine_verlet_update(struct engine *e);
+/* New helper: s->maxdx and the per-cell maxdx (cell_maxdx may be NULL). */
+int engine_getmaxdx(struct engine *e, double *maxdx, double *cell_maxdx);
+/* New helper: by particle id, the cell-relative position (x, 3 per id) and
+   the cell id (cell, may be NULL), as stored; returns the count. */
+int engine_getlocal(struct engine *e, double *x, int *cell, int N);
 int engine_shuffle(struct engine *e);
 /* space_prepare(&e->s): zero forces and energies before a pass (new helper). */
 int engine_prepare(struct engine *e);

@@ -369,6 +369,14 @@ __global__ __launch_bounds__(64) void k_sort_rank(struct nb_params P, const int 
 
 /* ------------------------------------------------------------ k_nonbond */
 
+/* Window bound of a pair task (runner_dopair.c:428): 2 + dscale (cutoff +
+   maxdx_i + maxdx_j) in FP64, left to right, no contraction, truncated; i
+   and j in task order. */
+__device__ __forceinline__ int pair_dmaxdist(double dscale, double cutoff, double mi, double mj) {
+#pragma clang fp contract(off)
+    return (int)(2 + dscale * (cutoff + mi + mj));
+}
+
 struct nb_out {
     float4 *fhi, *flo;
     double *cell_acc;
@@ -619,7 +627,13 @@ __device__ __forceinline__ void nonbond_body(
                    staged key k' = key (owner is i) or 0xffff - key (owner is
                    j), both read k' < kmax, and k' ascends along the staged
                    order, so a lane's window ends at its first failing record. */
-                const int kmax = orient ? P.dmaxdist - (myk + dns - 0xffff) : P.dmaxdist - (dns - myk);
+                /* With pseudo-Verlet sort reuse the window widens by both
+                   cells' displacement since the last sort. */
+                const int dmd = (P.maxdx != nullptr && !self)
+                                    ? pair_dmaxdist(P.dscale, P.cutoff, P.maxdx[orient ? en.x : c],
+                                                    P.maxdx[orient ? c : en.x])
+                                    : P.dmaxdist;
+                const int kmax = orient ? dmd - (myk + dns - 0xffff) : dmd - (dns - myk);
 
                 float lfx = 0.f, lfy = 0.f, lfz = 0.f, le = 0.f, lv = 0.f;
                 unsigned int lnp = 0, lnref = 0;
@@ -702,14 +716,14 @@ __device__ __forceinline__ void nonbond_body(
                                         lv -= fa * r2a;
                                         lnp++;
                                         lnref += (VK == 0 || !P.proj ||
-                                                  (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                                  (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
                                     }
                                     if (cb) {
                                         le += eb;
                                         lv -= fb * r2b;
                                         lnp++;
                                         lnref += (VK == 0 || !P.proj ||
-                                                  (live_ref && ((int)rec_key(wb) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                                  (live_ref && ((int)rec_key(wb) + en.z - myk) < dmd)) ? 1u : 0u;
                                     }
                                 }
                             }
@@ -742,7 +756,7 @@ __device__ __forceinline__ void nonbond_body(
                                     lv -= fa * r2a;
                                     lnp++;
                                     lnref += (VK == 0 || !P.proj ||
-                                              (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                              (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
                                 }
                             }
                             /* Dense list: this lane's hits at [off, off + nd). */
@@ -773,7 +787,7 @@ __device__ __forceinline__ void nonbond_body(
                                     lv -= fa * r2a;
                                     lnp++;
                                     lnref += (VK == 0 || !P.proj ||
-                                              (live_ref && ((int)rec_key(wa) + en.z - myk) < P.dmaxdist)) ? 1u : 0u;
+                                              (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
                                 }
                             }
                             wave_lds_sync();
@@ -1078,6 +1092,7 @@ static int gpu_alloc_parts(struct gpu_state *g, int n) {
     size_t N;
     if (n <= g->nparts_cap) return 0;
     gpu_free_pass(g);
+    g->sort_valid = 0;
     N = (size_t)n + n / 8 + 64;
     HIPCHECK(hipMalloc(&g->d_x, sizeof(double) * 3 * N));
     HIPCHECK(hipMalloc(&g->d_type, sizeof(int) * N));
@@ -1148,6 +1163,9 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_tot_cnt);
     (void)hipFree(g->d_queue);
     (void)hipHostFree(g->h_count);
+    (void)hipFree(g->d_cell_maxdx);
+    (void)hipHostFree(g->h_cell_maxdx);
+    free(g->h_sort_count);
     (void)hipHostFree(g->h_offset);
     (void)hipHostFree(g->h_cnt_off);
     (void)hipHostFree(g->h_tot);
@@ -1304,6 +1322,8 @@ extern "C" int engine_gpu_load(struct engine *e) {
         P->bias = sqrt(s->h[0] * s->h[0] + s->h[1] * s->h[1] + s->h[2] * s->h[2]);
         P->dscale = (double)32767 / (2 * P->bias);
         P->dmaxdist = (int)(2 + P->dscale * (s->cutoff + 0.0 + 0.0));
+        P->cutoff = s->cutoff;
+        P->maxdx = NULL;
         P->cutoff2 = s->cutoff * s->cutoff;
         for (k = 0; k < 3; k++) {
             P->dim[k] = s->dim[k];
@@ -1440,6 +1460,11 @@ extern "C" int engine_gpu_load(struct engine *e) {
     HIPCHECK(hipMalloc(&g->d_tot_cnt, sizeof(unsigned long long) * 4));
     HIPCHECK(hipMalloc(&g->d_queue, sizeof(int) * (MD_NQUEUE + 1)));
     HIPCHECK(hipHostMalloc((void **)&g->h_count, sizeof(int) * s->nr_cells, hipHostMallocDefault));
+    HIPCHECK(hipMalloc(&g->d_cell_maxdx, sizeof(double) * s->nr_cells));
+    HIPCHECK(hipHostMalloc((void **)&g->h_cell_maxdx, sizeof(double) * s->nr_cells, hipHostMallocDefault));
+    if ((g->h_sort_count = (int *)calloc(s->nr_cells, sizeof(int))) == NULL)
+        return gerror_(engine_err_malloc, "malloc failed");
+    g->sort_valid = 0;
     HIPCHECK(hipHostMalloc((void **)&g->h_offset, sizeof(int) * (s->nr_cells + 1), hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_cnt_off, sizeof(int2) * s->nr_cells, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_tot, sizeof(double) * 4, hipHostMallocDefault));
@@ -1505,7 +1530,7 @@ int gpu_upload(struct engine *e) {
 /* Launch the kernels of one pass; optional events around each. */
 /* One device pass on the resident particles; mlp selects the surrogate
    mode of the pair kernel. */
-int gpu_launch(struct gpu_state *g, int timed, int mlp) {
+int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget */
     const int kern = mlp ? 1 : (g->ncells >= 4 * 16 * g->ncu ? 2 : 0);
     struct nb_params P = g->P;
@@ -1541,6 +1566,7 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp) {
         g->lds_nb[kern] = lds;
     }
     if (mlp && g->mlp_np == 0) return gerror_(engine_err_range, "engine_flag_mlp without engine_mlp_init");
+    if (!sort && !g->sort_valid) return gerror_(engine_err_range, "no sort lists to reuse");
 
     if (timed) HIPCHECK(hipEventRecord(g->ev[0], g->stream));
     nb = (g->nids + 255) / 256;
@@ -1548,7 +1574,10 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp) {
     nb = ((g->nparts > g->ncells ? g->nparts : g->ncells) + 255) / 256;
     hipLaunchKernelGGL(k_pack, dim3(nb), dim3(256), 0, g->stream, g->nparts, g->d_x, g->d_type, g->d_id, g->ncells,
                        g->d_offset, g->d_xf, g->d_index_of_id, g->d_cell_of, g->d_queue);
-    if (g->maxcount <= MD_SORT_LDS_MAX) {
+    if (!sort) {
+        /* pseudo-Verlet step between rebuilds: the sort lists of the last
+           sort stay (runner.c:773) */
+    } else if (g->maxcount <= MD_SORT_LDS_MAX) {
         const int maxn = g->maxcount > 0 ? g->maxcount : 1;
         const size_t sh = sizeof(double) * 3 * maxn + sizeof(unsigned int) * 13 * maxn +
                           sizeof(int) * 13 * 2 * MD_SORT_STACK;
@@ -1559,6 +1588,7 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp) {
         hipLaunchKernelGGL(k_sort, dim3(nb), dim3(256), 0, g->stream, P, g->d_count, g->d_offset, g->d_sortflags,
                            g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE);
     }
+    if (sort) g->sort_valid = 1;
     if (timed) HIPCHECK(hipEventRecord(g->ev[1], g->stream));
     struct nb_out O;
     O.fhi = g->d_fhi;
@@ -1626,6 +1656,37 @@ extern "C" int engine_gpu_sync(struct engine *e, unsigned int what) {
     return engine_err_ok;
 }
 
+/* Pseudo-Verlet (engine_flag_verlet*): the pass sorts only on a rebuild
+   step (runner.c:773) and widens each pair window by the cells' maxdx
+   (runner_dopair.c:428).  Sets *sort and the device per-cell maxdx; a pass
+   that reuses the sort lists needs the layout of the last sort. */
+static int gpu_verlet(struct engine *e, struct gpu_state *g, int *sort) {
+    struct space *s = &e->s;
+    int cid;
+    if (!(e->flags & engine_flag_verlet)) {
+        g->P.maxdx = NULL;
+        *sort = 1;
+        return engine_err_ok;
+    }
+    for (cid = 0; cid < s->nr_cells; cid++) g->h_cell_maxdx[cid] = s->cells[cid].maxdx;
+    HIPCHECK(hipMemcpyAsync(g->d_cell_maxdx, g->h_cell_maxdx, sizeof(double) * s->nr_cells, hipMemcpyHostToDevice,
+                            g->stream));
+    g->P.maxdx = g->d_cell_maxdx;
+    *sort = s->verlet_rebuild != 0;
+    if (!*sort) {
+        if (!g->sort_valid) return gerror_(engine_err_range, "no sort lists to reuse");
+        for (cid = 0; cid < s->nr_cells; cid++)
+            if (s->cells[cid].count != g->h_sort_count[cid])
+                return gerror_(engine_err_range, "cells changed since the last sort");
+    }
+    return engine_err_ok;
+}
+
+/* Counts of the layout the sort lists belong to. */
+static void gpu_sorted(struct engine *e, struct gpu_state *g) {
+    for (int cid = 0; cid < e->s.nr_cells; cid++) g->h_sort_count[cid] = e->s.cells[cid].count;
+}
+
 extern "C" int engine_nonbond_gpu(struct engine *e) {
     struct gpu_state *g;
     struct space *s;
@@ -1636,8 +1697,11 @@ extern "C" int engine_nonbond_gpu(struct engine *e) {
     s = &e->s;
     HIPCHECK(hipSetDevice(g->device));
     if (gpu_upload(e) < 0) return engine_err_gpu;
+    int sort = 1;
+    if (gpu_verlet(e, g, &sort) < 0) return engine_err_range;
     HIPCHECK(hipMemsetAsync(g->d_queue + MD_NQUEUE, 0, sizeof(int), g->stream));
-    if (gpu_launch(g, 1, (e->flags & engine_flag_mlp) != 0) < 0) return engine_err_gpu;
+    if (gpu_launch(g, 1, (e->flags & engine_flag_mlp) != 0, sort) < 0) return engine_err_gpu;
+    if (sort) gpu_sorted(e, g);
     HIPCHECK(hipMemcpyAsync(g->h_fhi, g->d_fhi, sizeof(float4) * g->nparts, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipMemcpyAsync(g->h_flo, g->d_flo, sizeof(float4) * g->nparts, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipMemcpyAsync(g->h_tot, g->d_tot, sizeof(double) * 3, hipMemcpyDeviceToHost, g->stream));
@@ -1678,11 +1742,13 @@ extern "C" int engine_gpu_time(struct engine *e, int nsteps, int nwarm, double *
     g = (struct gpu_state *)e->gpu;
     if (g == NULL || g->nparts_cap == 0) return gerror_(engine_err_nogpu, "no resident particles");
     HIPCHECK(hipSetDevice(g->device));
+    /* the sort of a pass that engine_step would make, on the resident data */
+    const int sort = !(e->flags & engine_flag_verlet) || e->s.verlet_rebuild || !g->sort_valid;
     for (k = 0; k < nwarm; k++)
-        if (gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0) < 0) return engine_err_gpu;
+        if (gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0, sort) < 0) return engine_err_gpu;
     HIPCHECK(hipStreamSynchronize(g->stream));
     for (k = 0; k < nsteps; k++) {
-        if (gpu_launch(g, 1, (e->flags & engine_flag_mlp) != 0) < 0) return engine_err_gpu;
+        if (gpu_launch(g, 1, (e->flags & engine_flag_mlp) != 0, sort) < 0) return engine_err_gpu;
         HIPCHECK(hipEventSynchronize(g->ev[3]));
         HIPCHECK(hipEventElapsedTime(&t, g->ev[0], g->ev[1]));
         acc[0] += t;

@@ -33,6 +33,10 @@ struct nb_params {
     double dscale, bias, cutoff2;
     double dim[3], h[3];
     float rc2f, band;
+    double cutoff;
+    /* Pseudo-Verlet: per-cell maxdx widening each pair window
+       (runner_dopair.c:428), or NULL (no widening: dmaxdist). */
+    const double *maxdx;
     int maxn, maxnbr;
     int ptab_bytes, coef_bytes; /* sizes of the table headers and coefficients */
     /* dynamic LDS layout (byte offsets) */
@@ -75,6 +79,11 @@ struct gpu_state {
     double *d_cell_acc, *d_tot;
     unsigned long long *d_cell_cnt, *d_tot_cnt;
     int *d_queue;
+    /* pseudo-Verlet: per-cell maxdx on the device; whether the sort lists
+       are those of the last sort of the current layout, and its counts */
+    double *d_cell_maxdx, *h_cell_maxdx;
+    int sort_valid;
+    int *h_sort_count;
     /* pinned host staging */
     double *h_x;
     int *h_type, *h_id, *h_count, *h_offset;
@@ -114,10 +123,11 @@ static inline struct mlp_net gpu_mlp_net(const struct gpu_state *g) {
 void gpu_mlp_free(struct gpu_state *g);
 
 /* engine_gpu.hip: pack the host cells and upload them (no kernels); launch
-   one pass on the resident particles (mlp: surrogate mode), with events
-   around each kernel when timed. */
+   one pass on the resident particles (mlp: surrogate mode; sort == 0 keeps
+   the sort lists of the last sort), with events around each kernel when
+   timed. */
 int gpu_upload(struct engine *e);
-int gpu_launch(struct gpu_state *g, int timed, int mlp);
+int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort);
 
 #endif /* GPU_STATE_H */
 

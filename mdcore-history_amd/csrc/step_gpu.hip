@@ -493,6 +493,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     if (g == NULL) return gerror_(engine_err_nogpu, "engine_gpu_load was not called");
     if (nsteps < 0) return gerror_(engine_err_range, "negative step count");
     if (nsteps == 0) return engine_err_ok;
+    if (e->flags & engine_flag_verlet) return gerror_(engine_err_range, "pseudo-Verlet steps run through engine_step");
     s = &e->s;
     HIPCHECK(hipSetDevice(g->device));
     if (gpu_upload(e) < 0) return engine_err_gpu;
@@ -501,6 +502,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
         G.cdim[k] = s->cdim[k];
     }
     G.ncells = s->nr_cells;
+    g->P.maxdx = NULL;
     if (step_alloc(&b, (size_t)g->nparts_cap, g->ncells, g->max_type, nsteps) < 0)
         return gerror_(engine_err_gpu, "step buffers");
     {
@@ -537,7 +539,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
         }
         g->maxcount = maxc;
         g->P.maxn = maxc > 0 ? maxc : 1;
-        if ((rc = gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0)) < 0) break;
+        if ((rc = gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0, 1)) < 0) break;
         if ((rc = step_migrate(g, &b, G, e->dt, b.esteps + 4 * step)) < 0 ||
             (rc = step_status(g, &b, &maxc, &outside)) < 0)
             break;
@@ -551,5 +553,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     if (rc == engine_err_ok) rc = step_download(e, g, &b, nsteps, epot, ekin);
     (void)hipStreamSynchronize(g->stream);
     step_free(&b);
+    /* the last sort lists belong to the layout before the last migration */
+    g->sort_valid = 0;
     return rc;
 }

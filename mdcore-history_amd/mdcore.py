@@ -26,6 +26,8 @@ engine_flag_gpu = 4
 engine_flag_cuda = 4
 engine_flag_nullpart = 32768
 engine_flag_projwindow = 65536
+engine_flag_verlet = 16
+engine_flag_verlet_pseudo = 256
 engine_flag_mlp = 131072
 
 space_periodic_full = 7
@@ -81,6 +83,10 @@ def library():
         "engine_getenergies": (C.c_int, [C.c_void_p, _dp, _llp]),
         "engine_getsortlist": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up]),
         "engine_getcells": (C.c_int, [C.c_void_p, _ip, _ip]),
+        "engine_getlocal": (C.c_int, [C.c_void_p, _dp, _ip, C.c_int]),
+        "engine_getmaxdx": (C.c_int, [C.c_void_p, _dp, _dp]),
+        "engine_verlet_update": (C.c_int, [C.c_void_p]),
+        "engine_advance": (C.c_int, [C.c_void_p]),
         "engine_gettasks": (C.c_int, [C.c_void_p, _ip, C.c_int]),
         "engine_prepare": (C.c_int, [C.c_void_p]),
         "engine_gpu_setdevice": (C.c_int, [C.c_void_p, C.c_int]),
@@ -332,6 +338,26 @@ class Engine:
         ekin = np.zeros(max(nsteps, 1))
         check(self.lib.engine_gpu_steps(self.e, nsteps, ptr(epot), ptr(ekin)), "engine_gpu_steps")
         return epot[:nsteps], ekin[:nsteps]
+
+    def local(self):
+        """By particle id: cell-relative positions as stored, and cell ids."""
+        x = np.zeros(3 * max(self.N, 1))
+        cell = np.zeros(max(self.N, 1), dtype=np.int32)
+        check(self.lib.engine_getlocal(self.e, ptr(x), ptr(cell, C.c_int), self.N), "engine_getlocal")
+        return x[:3 * self.N].reshape(-1, 3), cell[:self.N]
+
+    def maxdx(self, ncells):
+        """(s->maxdx, per-cell maxdx, s->verlet_rebuild)."""
+        m = np.zeros(1)
+        cm = np.zeros(max(ncells, 1))
+        rb = check(self.lib.engine_getmaxdx(self.e, ptr(m), ptr(cm)), "engine_getmaxdx")
+        return float(m[0]), cm[:ncells], rb
+
+    def verlet_update(self):
+        check(self.lib.engine_verlet_update(self.e), "engine_verlet_update")
+
+    def advance(self):
+        check(self.lib.engine_advance(self.e), "engine_advance")
 
     def cells(self, ncells):
         counts = np.zeros(ncells, dtype=np.int32)

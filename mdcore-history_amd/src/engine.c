@@ -206,11 +206,20 @@ int engine_shuffle(struct engine *e) {
     return engine_err_ok;
 }
 
+static int verlet_store(struct space *s);
+
 int engine_start(struct engine *e, int nr_runners, int nr_queues) {
     (void)nr_queues;
     if (e == NULL) return eerror_(engine_err_null);
     if (!(e->flags & engine_flag_gpu)) return eerror_(engine_err_nogpu);
     e->nr_runners = nr_runners;
+    /* Pseudo-Verlet: shuffle, reference positions, rebuild on the first
+       step (engine.c:1712-1737). */
+    if (e->flags & engine_flag_verlet) {
+        if (engine_shuffle(e) < 0) return eerror_(engine_err);
+        if (verlet_store(&e->s) < 0) return eerror_(engine_err);
+        e->s.verlet_rebuild = 1;
+    }
     if (engine_gpu_load(e) < 0) return eerror_(engine_err_gpu);
     return engine_err_ok;
 }
@@ -222,12 +231,114 @@ int engine_nonbond_eval(struct engine *e) {
     return engine_err_ok;
 }
 
-/* Kick-drift and migration, then collect the cell energies
-   (engine.c:1945-2052, non-Verlet branch). */
+/* Store the current positions as the reference of the displacement test
+   (engine.c:286-297, 1717-1730). */
+static int verlet_store(struct space *s) {
+    int cid, pid, k;
+    for (cid = 0; cid < s->nr_real; cid++) {
+        struct cell *c = &s->cells[s->cid_real[cid]];
+        if (c->oldx == NULL || c->oldx_size < c->count) {
+            free(c->oldx);
+            c->oldx_size = c->size + 20;
+            if ((c->oldx = (FPTYPE *)malloc(sizeof(FPTYPE) * 3 * c->oldx_size)) == NULL) {
+                c->oldx_size = 0;
+                return eerror_(engine_err_malloc);
+            }
+        }
+        for (pid = 0; pid < c->count; pid++)
+            for (k = 0; k < 3; k++) c->oldx[3 * pid + k] = c->parts[pid].x[k];
+    }
+    return engine_err_ok;
+}
+
+/* engine.c:193-315 (the OpenMP branch, which sets each cell's maxdx): the
+   largest displacement since the last rebuild, per cell and overall; a
+   rebuild (shuffle, new reference positions, re-sort) once 2 maxdx exceeds
+   the skin min(h) - rc or maxdx exceeds rc / 4.  As in the reference, a
+   rebuild leaves the cells' maxdx as they were and zeroes only s->maxdx. */
+int engine_verlet_update(struct engine *e) {
+    struct space *s;
+    int cid, pid, k;
+    double dx, w, maxdx = 0.0, skin;
+    if (e == NULL) return eerror_(engine_err_null);
+    if (!(e->flags & engine_flag_verlet)) return engine_err_ok;
+    s = &e->s;
+    skin = fmin(s->h[0], fmin(s->h[1], s->h[2])) - s->cutoff;
+    if (!s->verlet_rebuild) {
+        for (cid = 0; cid < s->nr_real; cid++) {
+            struct cell *c = &s->cells[s->cid_real[cid]];
+            double lmaxdx = 0.0;
+            for (pid = 0; pid < c->count; pid++) {
+                for (dx = 0.0, k = 0; k < 3; k++) {
+                    w = c->parts[pid].x[k] - c->oldx[3 * pid + k];
+                    dx += w * w;
+                }
+                lmaxdx = fmax(dx, lmaxdx);
+            }
+            c->maxdx = sqrt(lmaxdx);
+            maxdx = fmax(lmaxdx, maxdx);
+        }
+        maxdx = sqrt(maxdx);
+        s->verlet_rebuild = (2.0 * maxdx > skin) || (maxdx > s->cutoff * engine_maxdxratio);
+    }
+    if (s->verlet_rebuild) {
+        if (engine_shuffle(e) < 0) return eerror_(engine_err);
+        if (verlet_store(s) < 0) return eerror_(engine_err);
+        s->maxdx = 0.0;
+    } else
+        s->maxdx = maxdx;
+    return engine_err_ok;
+}
+
+int engine_getmaxdx(struct engine *e, double *maxdx, double *cell_maxdx) {
+    int cid;
+    if (e == NULL) return eerror_(engine_err_null);
+    if (maxdx) *maxdx = e->s.maxdx;
+    if (cell_maxdx)
+        for (cid = 0; cid < e->s.nr_cells; cid++) cell_maxdx[cid] = e->s.cells[cid].maxdx;
+    return e->s.verlet_rebuild;
+}
+
+int engine_getlocal(struct engine *e, double *x, int *cell, int N) {
+    int cid, k, m, n = 0;
+    if (e == NULL || x == NULL) return eerror_(engine_err_null);
+    for (cid = 0; cid < e->s.nr_cells; cid++) {
+        struct cell *c = &e->s.cells[cid];
+        for (k = 0; k < c->count; k++) {
+            const int id = c->parts[k].id;
+            if (id < 0 || id >= N) return eerror_(engine_err_range);
+            for (m = 0; m < 3; m++) x[3 * id + m] = c->parts[k].x[m];
+            if (cell) cell[id] = cid;
+            n++;
+        }
+    }
+    return n;
+}
+
+/* Kick-drift, then migration (engine.c:1945-2052).  With the pseudo-Verlet
+   flag the particles stay in their cells until the next rebuild, so only
+   the kick-drift is done (the Verlet branch, 1961-1985). */
 int engine_advance(struct engine *e) {
     struct space *s = &e->s;
     int cid, pid, k, delta[3];
     double epot = 0.0, dt = e->dt, w;
+    if (e->flags & engine_flag_verlet) {
+        for (cid = 0; cid < s->nr_real; cid++) {
+            struct cell *c = &s->cells[s->cid_real[cid]];
+            epot += c->epot;
+            for (pid = 0; pid < c->count; pid++) {
+                struct part *p = &c->parts[pid];
+                w = dt * e->types[p->type].imass;
+                for (k = 0; k < 3; k++) {
+                    p->v[k] += p->f[k] * w;
+                    p->x[k] += dt * p->v[k];
+                }
+            }
+        }
+        s->epot_nonbond += epot;
+        s->epot += epot;
+        return engine_err_ok;
+    }
     for (cid = 0; cid < s->nr_real; cid++) {
         struct cell *c = &s->cells[s->cid_real[cid]];
         epot += c->epot;
@@ -269,7 +380,10 @@ int engine_step(struct engine *e) {
     if (e == NULL) return eerror_(engine_err_null);
     e->time += 1;
     t = wall_seconds();
-    if (engine_shuffle(e) < 0) return eerror_(engine_err_space);
+    if (e->flags & engine_flag_verlet) {
+        if (engine_verlet_update(e) < 0) return eerror_(engine_err);
+    } else if (engine_shuffle(e) < 0)
+        return eerror_(engine_err_space);
     e->timers[engine_timer_shuffle] = wall_seconds() - t;
     t = wall_seconds();
     if (space_prepare(&e->s) != space_err_ok) return eerror_(engine_err_space);
@@ -277,6 +391,7 @@ int engine_step(struct engine *e) {
     t = wall_seconds();
     if (engine_nonbond_eval(e) < 0) return eerror_(engine_err);
     e->timers[engine_timer_nonbond] = wall_seconds() - t;
+    if ((e->flags & engine_flag_verlet) && e->s.verlet_rebuild) e->s.verlet_rebuild = 0;
     t = wall_seconds();
     if (engine_advance(e) < 0) return eerror_(engine_err);
     e->timers[engine_timer_advance] = wall_seconds() - t;

@@ -464,6 +464,7 @@ void space_free(struct space *s) {
         free(s->cells[cid].parts);
         free(s->cells[cid].sortlist);
         free(s->cells[cid].incomming);
+        free(s->cells[cid].oldx);
     }
     free(s->cells);
     free(s->cid_real);

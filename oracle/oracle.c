@@ -70,6 +70,11 @@ struct or_system {
     int max_type, ntab, *pind, *tn, *toff;
     double *talpha, *tc;
     int nexcl, *excl;
+    /* pseudo-Verlet state: per-cell maxdx widening the pair windows
+       (runner_dopair.c:428) and whether or_nonbond keeps the sort lists of
+       the last sort (runner.c:773: cells are re-sorted only on a rebuild) */
+    double *maxdx;
+    int keep_sort;
     /* per-thread scratch of or_nonbond, kept between calls */
     int nbuf;
     double *fbuf, *ebuf;
@@ -218,6 +223,7 @@ or_system *or_create(const double *origin, const double *dim, const double *L, d
     s->cx = (double **)calloc(nc, sizeof(double *));
     s->sl = (unsigned int **)calloc(nc, sizeof(unsigned int *));
     s->sflags = (unsigned int *)calloc(nc, sizeof(unsigned int));
+    s->maxdx = (double *)calloc(nc, sizeof(double));
 
     /* Tasks in the reference's order (periodic in all directions). */
     s->ntask = 0;
@@ -437,6 +443,7 @@ void or_free(or_system *s) {
     free(s->cnt);
     free(s->cap);
     free(s->sflags);
+    free(s->maxdx);
     free(s->corg);
     free(s->cloc);
     free(s->type);
@@ -523,7 +530,8 @@ static void do_pair(const or_system *s, int ci, int cj, int sid, int mode, doubl
     const double *xi = s->cx[ci], *xj = s->cx[cj];
     double *fi = &f[3 * foff[ci]], *fj = &f[3 * foff[cj]];
     if (ni == 0 || nj == 0) return;
-    dmaxdist = (int)(2 + s->dscale * (s->cutoff + 0.0 + 0.0));
+    /* runner_dopair.c:428, with the task's cells in task order */
+    dmaxdist = (int)(2 + s->dscale * (s->cutoff + s->maxdx[ci] + s->maxdx[cj]));
     (void)or_orient(s, ci, cj, sh);
     if (mode == OR_WINDOW_REF)
         dnshift = (int)(s->dscale * sqrt(sh[0] * sh[0] + sh[1] * sh[1] + sh[2] * sh[2]));
@@ -590,7 +598,8 @@ static void *worker_main(void *arg) {
     if (w->phase == 0) {
         memset(w->f, 0, sizeof(double) * n3);
         memset(w->cepot, 0, sizeof(double) * s->ncells);
-        while ((t = __atomic_fetch_add(w->next, 1, __ATOMIC_RELAXED)) < s->ncells) sort_cell(s, t);
+        if (!s->keep_sort)
+            while ((t = __atomic_fetch_add(w->next, 1, __ATOMIC_RELAXED)) < s->ncells) sort_cell(s, t);
         return NULL;
     }
     if (w->phase == 2) {
@@ -818,6 +827,18 @@ int or_cell(const or_system *s, int c, int *count, int *ids, unsigned int *sortf
     if (ids) memcpy(ids, s->cid[c], sizeof(int) * s->cnt[c]);
     if (sortflags) *sortflags = s->sflags[c];
     return s->cnt[c];
+}
+
+int or_set_verlet(or_system *s, const double *xrel, const double *cell_maxdx, int keep_sort) {
+    int id, k, c;
+    if (s == NULL) return -1;
+    if (xrel)
+        for (id = 0; id < s->N; id++)
+            for (k = 0; k < 3; k++) s->cx[s->cell_of[id]][3 * s->slot_of[id] + k] = xrel[3 * id + k];
+    if (cell_maxdx)
+        for (c = 0; c < s->ncells; c++) s->maxdx[c] = cell_maxdx[c];
+    s->keep_sort = keep_sort;
+    return 0;
 }
 
 int or_sortlist(const or_system *s, int c, int sid, unsigned int *out) {

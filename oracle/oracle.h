@@ -51,6 +51,13 @@ int or_nonbond(or_system *s, int mode, int nthreads, double *f, double *energies
    counts every pair with r^2 <= rc^2 and a potential, and sums FP64 forces. */
 int or_bruteforce(or_system *s, double *f, double *energies, long long *count);
 
+/* Pseudo-Verlet steps (engine_flag_verlet_pseudo): move the particles
+   without changing their cells (xrel: cell-relative positions by id, may lie
+   outside the cell; NULL keeps them), set each cell's maxdx (NULL keeps
+   them), and with keep_sort != 0 make or_nonbond reuse the sort lists of the
+   last sort instead of sorting (runner.c:773, runner_dopair.c:428). */
+int or_set_verlet(or_system *s, const double *xrel, const double *cell_maxdx, int keep_sort);
+
 /* Introspection. */
 int or_ncells(const or_system *s);
 int or_cell(const or_system *s, int cid, int *count, int *ids, unsigned int *sortflags);

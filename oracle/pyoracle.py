@@ -56,6 +56,7 @@ def lib():
             "or_ntasks": (C.c_int, [C.c_void_p, _ip, _ip]),
             "or_task": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip]),
             "or_dscale": (C.c_double, [C.c_void_p]),
+            "or_set_verlet": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
             "or_sort_desc": (None, [_up, C.c_int]),
             "or_key": (C.c_uint, [_dp, _dp, C.c_double, C.c_double]),
             "or_eval": (None, [C.c_int, _dp, _dp, C.c_double, _dp, _dp]),
@@ -182,6 +183,13 @@ class Oracle:
         if lib().or_bruteforce(self.h, _p(f), _p(en), _p(cnt, C.c_longlong)) < 0:
             raise RuntimeError("brute force needs at least 2*span+1 cells per axis")
         return {"f": f.reshape(-1, 3), "epot": en[0], "virial": en[2], "pairs": int(cnt[0])}
+
+    def set_verlet(self, xrel=None, cell_maxdx=None, keep_sort=True):
+        """Pseudo-Verlet state: cell-relative positions by id (cells unchanged),
+        per-cell maxdx, and whether nonbond() keeps the last sort lists."""
+        xr = None if xrel is None else np.ascontiguousarray(xrel, dtype=np.float64).reshape(-1)
+        md = None if cell_maxdx is None else np.ascontiguousarray(cell_maxdx, dtype=np.float64)
+        lib().or_set_verlet(self.h, _p(xr), _p(md), 1 if keep_sort else 0)
 
     def sort(self):
         lib().or_sort(self.h)

@@ -9,7 +9,10 @@
   1M-atom geometry (62 pair tasks per cell);
 - pair loops: all-pairs brute force (the projected window finds every pair,
   the reference window misses some on span-2 cells), Newton's third law,
-  translation invariance, threaded == serial, exact exclusion cancellation.
+  translation invariance, threaded == serial, exact exclusion cancellation;
+- sort reuse (pseudo-Verlet): with stale sort lists the widened projected
+  window still finds every pair brute force finds, and the unwidened one
+  does not.
 """
 import ctypes as C
 
@@ -233,3 +236,42 @@ def test_reference_window_miss_fraction_matches_probe():
     prj = o.nonbond(po.OR_WINDOW_PROJ, 4)["pairs"]
     miss = 1.0 - ref / prj
     assert 0.0210 <= miss <= 0.0228, miss
+
+
+def test_sort_reuse_window_widening():
+    """Pseudo-Verlet sort reuse (runner.c:773: no re-sort between rebuilds;
+    runner_dopair.c:428: the window widened by both cells' maxdx).  After the
+    particles move up to 0.06 nm (within half the 0.13 nm skin) without a
+    re-sort, the projected window still finds every in-cutoff pair when
+    widened by each cell's largest displacement, and misses some without the
+    widening; a fresh sort needs none.  Brute force on the moved positions is
+    the judge."""
+    sysm = systems.argon(6, jitter=0.05)  # 864 atoms, 3^3 cells of 1.13 nm
+    o = po.from_system(sysm)
+    o.nonbond(po.OR_WINDOW_PROJ)  # sorts at the start positions
+    nc, cdim = o.ncells, 3
+    h = sysm.dim / cdim
+    cell = np.empty(sysm.N, np.int64)
+    for cid in range(nc):
+        ids, _ = o.cell(cid)
+        cell[ids] = cid
+    loc = np.stack([cell // (cdim * cdim), (cell // cdim) % cdim, cell % cdim], 1)
+    xrel0 = np.mod(sysm.x, sysm.dim) - loc * h
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(sysm.N, 3))
+    d *= (0.06 * rng.uniform(0.0, 1.0, sysm.N) / np.linalg.norm(d, axis=1))[:, None]
+    maxdx = np.zeros(nc)
+    np.maximum.at(maxdx, cell, np.linalg.norm(d, axis=1))
+    o.set_verlet(xrel0 + d, maxdx, keep_sort=True)
+    reuse = o.nonbond(po.OR_WINDOW_PROJ)
+    bf = o.bruteforce()
+    assert reuse["pairs"] == bf["pairs"]
+    assert np.abs(reuse["f"] - bf["f"]).max() <= 1e-9 * np.abs(bf["f"]).max()
+    o.set_verlet(None, np.zeros(nc), keep_sort=True)
+    narrow = o.nonbond(po.OR_WINDOW_PROJ)
+    assert narrow["pairs"] < bf["pairs"]
+    o.set_verlet(None, None, keep_sort=False)
+    fresh = o.nonbond(po.OR_WINDOW_PROJ)
+    assert fresh["pairs"] == bf["pairs"]
+    print(f"pairs: brute force {bf['pairs']}, reused sort widened {reuse['pairs']}, "
+          f"not widened {narrow['pairs']}")
