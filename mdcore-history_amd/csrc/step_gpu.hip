@@ -321,16 +321,124 @@ __global__ __launch_bounds__(128) void k_arrive(struct step_geom G, const int *_
     if (outside) st[1] = 1;
 }
 
+/* Pseudo-Verlet advance, one wave per cell: with kick, v += f dt / m and
+   x += dt v in place (engine_advance's Verlet branch, engine.c:1961-1985:
+   no migration) and the cell's kinetic energy; then the largest squared
+   displacement from the rebuild positions, ((0 + w0^2) + w1^2) + w2^2 with
+   w = x - oldx as in engine_verlet_update (engine.c:222-233). */
+__global__ __launch_bounds__(64) void k_advance_verlet(int kick, double dt, const double *__restrict__ mass,
+                                                       const double *__restrict__ imass,
+                                                       const float4 *__restrict__ fhi, const float4 *__restrict__ flo,
+                                                       const int *__restrict__ count, const int *__restrict__ offset,
+                                                       double *__restrict__ x, double *__restrict__ v,
+                                                       const int *__restrict__ type, const double *__restrict__ oldx,
+                                                       double *__restrict__ lmax2, double *__restrict__ cell_ke) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int off = offset[c], n = count[c];
+    double ke = 0.0, lm = 0.0;
+    for (int k = lane; k < n; k += 64) {
+        const int a = off + k;
+        double xa[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
+        if (kick) {
+            const int t = type[a];
+            const double w = dt * imass[t];
+            const float4 hi = fhi[a], lo = flo[a];
+            const double f[3] = {0.0 + ((double)hi.x + (double)lo.x), 0.0 + ((double)hi.y + (double)lo.y),
+                                 0.0 + ((double)hi.z + (double)lo.z)};
+            double vv[3];
+            for (int m = 0; m < 3; m++) {
+                vv[m] = v[3 * a + m] + f[m] * w;
+                xa[m] = xa[m] + dt * vv[m];
+                v[3 * a + m] = vv[m];
+                x[3 * a + m] = xa[m];
+            }
+            ke += 0.5 * mass[t] * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+        }
+        double dx = 0.0;
+        for (int m = 0; m < 3; m++) {
+            const double w = xa[m] - oldx[3 * a + m];
+            dx += w * w;
+        }
+        lm = fmax(dx, lm);
+    }
+    for (int s = 32; s > 0; s >>= 1) {
+        ke += __shfl_xor(ke, s, 64);
+        lm = fmax(lm, __shfl_xor(lm, s, 64));
+    }
+    if (lane == 0) {
+        cell_ke[c] = ke;
+        lmax2[c] = lm;
+    }
+}
+
+/* Pass 1 of the pseudo-Verlet sums: per block of MD_SCAN_CELLS cells, the
+   kinetic energy and the largest squared displacement. */
+__global__ __launch_bounds__(256) void k_verlet_red_a(int ncells, const double *__restrict__ cell_ke,
+                                                      const double *__restrict__ lmax2, double *__restrict__ bke,
+                                                      double *__restrict__ bmx) {
+    __shared__ double sk[256], sm[256];
+    const int t = threadIdx.x, c0 = blockIdx.x * MD_SCAN_CELLS + 4 * t;
+    double ke = 0.0, mx = 0.0;
+    for (int c = c0; c < c0 + 4 && c < ncells; c++) {
+        ke += cell_ke[c];
+        mx = fmax(lmax2[c], mx);
+    }
+    sk[t] = ke;
+    sm[t] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            sk[t] += sk[t + w];
+            sm[t] = fmax(sm[t], sm[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        bke[blockIdx.x] = sk[0];
+        bmx[blockIdx.x] = sm[0];
+    }
+}
+
+/* Pass 2, one thread: the step's energies (pass totals, kinetic) when es is
+   given, and the largest squared displacement into *vmax. */
+__global__ __launch_bounds__(64) void k_verlet_red_b(int nblocks, const double *__restrict__ bke,
+                                                     const double *__restrict__ bmx, const double *__restrict__ tot,
+                                                     double *__restrict__ es, double *__restrict__ vmax) {
+    if (threadIdx.x != 0) return;
+    double ke = 0.0, mx = 0.0;
+    for (int b = 0; b < nblocks; b++) {
+        ke += bke[b];
+        mx = fmax(bmx[b], mx);
+    }
+    *vmax = mx;
+    if (es) {
+        es[0] = tot[0];
+        es[1] = tot[1];
+        es[2] = tot[2];
+        es[3] = ke;
+    }
+}
+
+/* Each cell's maxdx = sqrt(largest squared displacement), engine.c:231. */
+__global__ __launch_bounds__(256) void k_cell_maxdx(int ncells, const double *__restrict__ lmax2,
+                                                    double *__restrict__ cell_maxdx) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncells) cell_maxdx[c] = sqrt(lmax2[c]);
+}
+
 /* ------------------------------------------------------------ host side */
 
 struct step_bufs {
     double *v, *x2, *v2, *mass, *imass, *cell_ke, *bke, *esteps;
+    double *oldx, *lmax2, *bmx, *vmax; /* pseudo-Verlet */
     int *id2, *type2, *slot, *qsrc, *qdest, *stay, *ndep, *narr, *bsum, *bmax, *st, *count2, *offset2;
     int nblocks;
 };
 
 static void step_free(struct step_bufs *b) {
-    double *d[] = {b->v, b->x2, b->v2, b->mass, b->imass, b->cell_ke, b->bke, b->esteps};
+    double *d[] = {b->v, b->x2, b->v2, b->mass, b->imass, b->cell_ke, b->bke, b->esteps,
+                   b->oldx, b->lmax2, b->bmx, b->vmax};
     int *i[] = {b->id2, b->type2, b->slot, b->qsrc, b->qdest, b->stay, b->ndep,
                 b->narr, b->bsum, b->bmax, b->st, b->count2, b->offset2};
     for (size_t k = 0; k < sizeof(d) / sizeof(d[0]); k++) (void)hipFree(d[k]);
@@ -357,6 +465,9 @@ static int step_alloc(struct step_bufs *b, size_t N, int ncells, int ntypes, int
         hipMalloc(&b->bsum, sizeof(int) * B) != hipSuccess || hipMalloc(&b->bmax, sizeof(int) * B) != hipSuccess ||
         hipMalloc(&b->st, sizeof(int) * 2) != hipSuccess || hipMalloc(&b->count2, sizeof(int) * C) != hipSuccess ||
         hipMalloc(&b->offset2, sizeof(int) * (C + 1)) != hipSuccess ||
+        hipMalloc(&b->oldx, sizeof(double) * 3 * N) != hipSuccess ||
+        hipMalloc(&b->lmax2, sizeof(double) * C) != hipSuccess || hipMalloc(&b->bmx, sizeof(double) * B) != hipSuccess ||
+        hipMalloc(&b->vmax, sizeof(double)) != hipSuccess ||
         hipMemset(b->narr, 0, sizeof(int) * C) != hipSuccess) {
         step_free(b);
         return -1;
@@ -414,7 +525,7 @@ static int step_status(struct gpu_state *g, struct step_bufs *b, int *maxc, int 
 /* Back to the host cells: layout, positions, velocities, the last pass's
    forces by particle id, and the energies engine_step leaves behind. */
 static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs *b, int nsteps, double *epot,
-                         double *ekin) {
+                         double *ekin, int verlet, double smaxdx) {
     struct space *s = &e->s;
     const int n = g->nparts, nc = g->ncells;
     const size_t M = (size_t)(n > 0 ? n : 1);
@@ -423,7 +534,10 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
     double *es = (double *)malloc(sizeof(double) * 4 * (size_t)nsteps);
     int *id = (int *)malloc(sizeof(int) * M), *type = (int *)malloc(sizeof(int) * M);
     int *count = (int *)malloc(sizeof(int) * (size_t)nc), *ioi = (int *)malloc(sizeof(int) * (size_t)g->nids);
-    if (x == NULL || v == NULL || es == NULL || id == NULL || type == NULL || count == NULL || ioi == NULL) {
+    double *ox = verlet ? (double *)malloc(sizeof(double) * 3 * M) : NULL;
+    double *cm = verlet ? (double *)malloc(sizeof(double) * (size_t)nc) : NULL;
+    if (x == NULL || v == NULL || es == NULL || id == NULL || type == NULL || count == NULL || ioi == NULL ||
+        (verlet && (ox == NULL || cm == NULL))) {
         rc = gerror_(engine_err_malloc, "malloc failed");
         goto done;
     }
@@ -439,6 +553,9 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
         hipMemcpyAsync(g->h_tot, g->d_tot, sizeof(double) * 3, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
         hipMemcpyAsync(g->h_tot_cnt, g->d_tot_cnt, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost,
                        g->stream) != hipSuccess ||
+        (verlet && (hipMemcpyAsync(ox, b->oldx, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                    hipMemcpyAsync(cm, g->d_cell_maxdx, sizeof(double) * nc, hipMemcpyDeviceToHost, g->stream) !=
+                        hipSuccess)) ||
         hipStreamSynchronize(g->stream) != hipSuccess) {
         rc = gerror_(engine_err_gpu, "download failed");
         goto done;
@@ -446,6 +563,28 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
     if (space_setcells(s, count, x, v, id, type) < 0) {
         rc = gerror_(engine_err_space, "rebuilding the cells failed");
         goto done;
+    }
+    if (verlet) {
+        /* reference positions per slot, each cell's maxdx, s->maxdx; the
+           last sort lists belong to this layout (no migration since) */
+        for (int cid = 0, a = 0; cid < nc; cid++) {
+            struct cell *c = &s->cells[cid];
+            if (c->oldx == NULL || c->oldx_size < c->count) {
+                free(c->oldx);
+                c->oldx_size = c->size + 20;
+                if ((c->oldx = (FPTYPE *)malloc(sizeof(FPTYPE) * 3 * c->oldx_size)) == NULL) {
+                    c->oldx_size = 0;
+                    rc = gerror_(engine_err_malloc, "malloc failed");
+                    goto done;
+                }
+            }
+            memcpy(c->oldx, &ox[3 * (size_t)a], sizeof(double) * 3 * c->count);
+            c->maxdx = cm[cid];
+            g->h_sort_count[cid] = c->count;
+            a += c->count;
+        }
+        s->maxdx = smaxdx;
+        s->verlet_rebuild = 0;
     }
     space_prepare(s);
     for (int cid = 0; cid < nc; cid++) {
@@ -472,6 +611,8 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
         if (ekin) ekin[k] = es[4 * k + 3];
     }
 done:
+    free(ox);
+    free(cm);
     free(x);
     free(v);
     free(es);
@@ -479,6 +620,97 @@ done:
     free(type);
     free(count);
     free(ioi);
+    return rc;
+}
+
+/* Largest squared displacement of the last k_advance_verlet. */
+static int verlet_status(struct gpu_state *g, struct step_bufs *b, double *vmax) {
+    if (hipMemcpyAsync(vmax, b->vmax, sizeof(double), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess)
+        return gerror_(engine_err_gpu, "advance failed");
+    return engine_err_ok;
+}
+
+/* The steps with pseudo-Verlet sort reuse: engine_verlet_update
+   (engine.c:193-315) at the start of each step, the pass sorting only on a
+   rebuild, the kick-only advance.  Positions, velocities and forces are
+   resident as in the plain loop; the rebuild positions (oldx, per slot) and
+   each cell's maxdx are too.  *smaxdx carries s->maxdx. */
+static int steps_verlet(struct engine *e, struct gpu_state *g, struct step_bufs *b, const struct step_geom &G,
+                        int nsteps, double *smaxdx) {
+    struct space *s = &e->s;
+    const double skin = fmin(s->h[0], fmin(s->h[1], s->h[2])) - s->cutoff;
+    const int nc = G.ncells, nbc = (nc + 255) / 256, mlp = (e->flags & engine_flag_mlp) != 0;
+    int rebuild = s->verlet_rebuild, rc = engine_err_ok, maxc = g->maxcount, outside = 0;
+    double vmax = 0.0;
+    g->P.maxdx = g->d_cell_maxdx;
+    {
+        /* each cell's maxdx, and the rebuild positions unless this step rebuilds */
+        double *cm = (double *)malloc(sizeof(double) * (size_t)nc);
+        double *ox = rebuild ? NULL : (double *)malloc(sizeof(double) * 3 * (size_t)(g->nparts > 0 ? g->nparts : 1));
+        if (cm == NULL || (!rebuild && ox == NULL)) rc = gerror_(engine_err_malloc, "malloc failed");
+        for (int cid = 0; cid < nc && rc == engine_err_ok; cid++) {
+            const struct cell *c = &s->cells[cid];
+            cm[cid] = c->maxdx;
+            if (rebuild) continue;
+            if (c->count > 0 && (c->oldx == NULL || c->oldx_size < c->count))
+                rc = gerror_(engine_err_range, "no rebuild positions (engine_start sets them)");
+            else if (c->count > 0)
+                memcpy(&ox[3 * (size_t)g->h_offset[cid]], c->oldx, sizeof(double) * 3 * c->count);
+        }
+        if (rc == engine_err_ok &&
+            (hipMemcpy(g->d_cell_maxdx, cm, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess ||
+             (!rebuild && hipMemcpy(b->oldx, ox, sizeof(double) * 3 * g->nparts, hipMemcpyHostToDevice) != hipSuccess)))
+            rc = gerror_(engine_err_gpu, "upload failed");
+        free(cm);
+        free(ox);
+        if (rc < 0) return rc;
+    }
+    if (!rebuild) {
+        /* displacements of the uploaded positions */
+        hipLaunchKernelGGL(k_advance_verlet, dim3(nc), dim3(64), 0, g->stream, 0, 0.0, b->mass, b->imass, g->d_fhi,
+                           g->d_flo, g->d_count, g->d_offset, g->d_x, b->v, g->d_type, b->oldx, b->lmax2, b->cell_ke);
+        hipLaunchKernelGGL(k_verlet_red_a, dim3(b->nblocks), dim3(256), 0, g->stream, nc, b->cell_ke, b->lmax2, b->bke,
+                           b->bmx);
+        hipLaunchKernelGGL(k_verlet_red_b, dim3(1), dim3(64), 0, g->stream, b->nblocks, b->bke, b->bmx, g->d_tot,
+                           (double *)NULL, b->vmax);
+        if ((rc = verlet_status(g, b, &vmax)) < 0) return rc;
+    }
+    for (int step = 0; step < nsteps; step++) {
+        if (!rebuild) {
+            /* engine_verlet_update without a pending rebuild */
+            hipLaunchKernelGGL(k_cell_maxdx, dim3(nbc), dim3(256), 0, g->stream, nc, b->lmax2, g->d_cell_maxdx);
+            const double maxdx = sqrt(vmax);
+            rebuild = (2.0 * maxdx > skin) || (maxdx > s->cutoff * engine_maxdxratio);
+            if (!rebuild) *smaxdx = maxdx;
+        }
+        if (rebuild) {
+            /* engine_shuffle, then the rebuild positions */
+            if ((rc = step_migrate(g, b, G, 0.0, NULL)) < 0 || (rc = step_status(g, b, &maxc, &outside)) < 0) break;
+            if (hipMemcpyAsync(b->oldx, g->d_x, sizeof(double) * 3 * g->nparts, hipMemcpyDeviceToDevice, g->stream) !=
+                hipSuccess) {
+                rc = gerror_(engine_err_gpu, "rebuild failed");
+                break;
+            }
+            *smaxdx = 0.0;
+            g->maxcount = maxc;
+            g->P.maxn = maxc > 0 ? maxc : 1;
+        }
+        if ((rc = gpu_launch(g, 0, mlp, rebuild)) < 0) break;
+        rebuild = 0;
+        hipLaunchKernelGGL(k_advance_verlet, dim3(nc), dim3(64), 0, g->stream, 1, e->dt, b->mass, b->imass, g->d_fhi,
+                           g->d_flo, g->d_count, g->d_offset, g->d_x, b->v, g->d_type, b->oldx, b->lmax2, b->cell_ke);
+        hipLaunchKernelGGL(k_verlet_red_a, dim3(b->nblocks), dim3(256), 0, g->stream, nc, b->cell_ke, b->lmax2, b->bke,
+                           b->bmx);
+        hipLaunchKernelGGL(k_verlet_red_b, dim3(1), dim3(64), 0, g->stream, b->nblocks, b->bke, b->bmx, g->d_tot,
+                           b->esteps + 4 * step, b->vmax);
+        if (hipGetLastError() != hipSuccess) {
+            rc = gerror_(engine_err_gpu, "advance failed");
+            break;
+        }
+        if ((rc = verlet_status(g, b, &vmax)) < 0) break;
+        e->time += 1;
+    }
     return rc;
 }
 
@@ -493,7 +725,6 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     if (g == NULL) return gerror_(engine_err_nogpu, "engine_gpu_load was not called");
     if (nsteps < 0) return gerror_(engine_err_range, "negative step count");
     if (nsteps == 0) return engine_err_ok;
-    if (e->flags & engine_flag_verlet) return gerror_(engine_err_range, "pseudo-Verlet steps run through engine_step");
     s = &e->s;
     HIPCHECK(hipSetDevice(g->device));
     if (gpu_upload(e) < 0) return engine_err_gpu;
@@ -529,10 +760,14 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
         free(imass);
         free(v);
     }
+    const int verlet = (e->flags & engine_flag_verlet) != 0;
+    double smaxdx = s->maxdx;
+    if (verlet && rc == engine_err_ok)
+        rc = steps_verlet(e, g, &b, G, nsteps, &smaxdx);
     /* engine_step: shuffle, pass, advance.  After an advance that left every
        particle inside its cell, the next shuffle moves nothing and is skipped. */
     outside = 1;
-    for (int step = 0; step < nsteps && rc == engine_err_ok; step++) {
+    for (int step = 0; step < nsteps && rc == engine_err_ok && !verlet; step++) {
         if (outside) {
             if ((rc = step_migrate(g, &b, G, 0.0, NULL)) < 0 || (rc = step_status(g, &b, &maxc, &outside)) < 0)
                 break;
@@ -550,10 +785,11 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
          hipStreamSynchronize(g->stream) != hipSuccess))
         rc = gerror_(engine_err_gpu, "step failed");
     if (rc == engine_err_ok && flag) rc = gerror_(engine_err_range, "sort stack overflow");
-    if (rc == engine_err_ok) rc = step_download(e, g, &b, nsteps, epot, ekin);
+    if (rc == engine_err_ok) rc = step_download(e, g, &b, nsteps, epot, ekin, verlet, smaxdx);
     (void)hipStreamSynchronize(g->stream);
     step_free(&b);
-    /* the last sort lists belong to the layout before the last migration */
-    g->sort_valid = 0;
+    /* without pseudo-Verlet the last sort lists belong to the layout before
+       the last migration */
+    if (!verlet || rc != engine_err_ok) g->sort_valid = 0;
     return rc;
 }

@@ -99,3 +99,77 @@ def test_rebuild_follows_the_skin_criterion():
     print(f"{nrebuild} rebuilds in 120 steps", flush=True)
     assert nrebuild >= 2
     eng.close()
+
+
+def mixture_moving():
+    sysm = systems.mixture(4000, seed=11)
+    return dataclasses.replace(sysm, v=md.gen_maxwell(sysm.masses[sysm.types], systems.KB * 300.0, 5))
+
+
+# (system, dt, steps): argon rebuilds every ~20 steps; the mixture's 0.6 nm
+# cells under a 1.2 nm cutoff have a negative skin and rebuild every step
+RESIDENT = {"argon_skin": (argon_skin, 0.004, 60), "mixture_4k": (mixture_moving, 0.0005, 12)}
+
+
+def kinetic(sysm, v):
+    return 0.5 * float(np.sum(sysm.masses[sysm.types][:, None] * v * v))
+
+
+def same_state(eh, ed, nc):
+    ch, ih = eh.cells(nc)
+    cd, idd = ed.cells(nc)
+    assert np.array_equal(ch, cd) and np.array_equal(ih, idd)
+    assert np.array_equal(eh.positions(), ed.positions())
+    assert np.array_equal(eh.velocities(), ed.velocities())
+    assert np.array_equal(eh.forces(), ed.forces())
+    assert eh.energies() == ed.energies()
+    mh, cmh, rbh = eh.maxdx(nc)
+    md_, cmd, rbd = ed.maxdx(nc)
+    assert mh == md_ and rbh == rbd
+    assert np.array_equal(cmh, cmd)
+
+
+@pytest.mark.parametrize("name", sorted(RESIDENT))
+def test_resident_verlet_steps_match_host_steps(name):
+    make, dt, n = RESIDENT[name]
+    sysm = make()
+    print(f"case {name}: N={sysm.N} dt={dt} steps={n}", flush=True)
+    flags = md.engine_flag_gpu | md.engine_flag_verlet_pseudo
+    eh = systems.build_engine(sysm, flags)
+    ed = systems.build_engine(sysm, flags)
+    nc = int(np.prod(np.floor(sysm.dim / sysm.L)))
+    for e in (eh, ed):
+        e.setdt(dt)
+    epot, ekin, rebuilds = [], [], 0
+    for _ in range(n):
+        eh.step()
+        epot.append(eh.energies()["epot"])
+        ekin.append(kinetic(sysm, eh.velocities()))
+        rebuilds += eh.maxdx(nc)[0] == 0.0
+    ep, ek = ed.gpu_steps(n)
+    assert np.array_equal(np.array(epot), ep)
+    np.testing.assert_allclose(ek, np.array(ekin), rtol=1e-12)
+    same_state(eh, ed, nc)
+    print(f"{name}: {rebuilds} rebuild steps in {n}", flush=True)
+    assert 2 <= rebuilds
+    eh.close()
+    ed.close()
+
+
+def test_resident_verlet_interleaves_with_host_steps():
+    sysm = argon_skin()
+    flags = md.engine_flag_gpu | md.engine_flag_verlet_pseudo
+    eh = systems.build_engine(sysm, flags)
+    ed = systems.build_engine(sysm, flags)
+    nc = int(np.prod(np.floor(sysm.dim / sysm.L)))
+    for e in (eh, ed):
+        e.setdt(0.004)
+    for _ in range(45):
+        eh.step()
+    ed.gpu_steps(17)
+    for _ in range(6):
+        ed.step()
+    ed.gpu_steps(22)
+    same_state(eh, ed, nc)
+    eh.close()
+    ed.close()
