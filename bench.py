@@ -24,7 +24,8 @@ passes timed with HIP events on the engine's stream.  Prints one JSON line.
   md_step       whole MD steps (shuffle, pass, kick-drift, migration) of the
                 1,048,576-atom argon crystal at 100 K, dt 2 fs: host-driven
                 engine_step (positions up, forces down, host integration)
-                against engine_gpu_steps (particles resident on the device)
+                against engine_gpu_steps (particles resident on the device),
+                and the resident steps with pseudo-Verlet sort reuse
 """
 import argparse
 import json
@@ -343,20 +344,36 @@ def nve_section(steps=1000, dt=0.002, equil=1000):
 
 def md_step_section(nsteps, host_steps=5, dt=0.002):
     """ms per whole MD step of the 1,048,576-atom argon crystal (fcc 64^3,
-    100 K): host-driven engine_step against engine_gpu_steps.  The resident
-    time is the wall time of one engine_gpu_steps call over nsteps steps,
-    including its one upload and one download."""
+    100 K): host-driven engine_step against engine_gpu_steps, with 1.0 nm
+    cells and re-sorting every step, and with pseudo-Verlet sort reuse in
+    1.13 nm cells (cell option 1.1 nm: 0.13 nm skin), and those cells
+    re-sorted every step."""
+    out = md_step_run(nsteps, host_steps, dt, 1.0, False)
+    out["sort_reuse"] = md_step_run(nsteps, 0, dt, 1.1, True)
+    out["same_cells_no_reuse"] = md_step_run(nsteps, 0, dt, 1.1, False)
+    return out
+
+
+def md_step_run(nsteps, host_steps, dt, cell, verlet):
+    """The resident time is the wall time of one engine_gpu_steps call over
+    nsteps steps, including its one upload and one download; a shorter call
+    separates the per-step cost from the per-call one."""
+    import dataclasses
     import numpy as np
     import mdcore as md
     import systems
-    sysm = systems.argon(64, T=100.0)
-    eng = systems.build_engine(sysm, md.engine_flag_gpu)
+    sysm = dataclasses.replace(systems.argon(64, T=100.0), L=np.full(3, cell))
+    flags = md.engine_flag_gpu | (md.engine_flag_verlet_pseudo if verlet else 0)
+    eng = systems.build_engine(sysm, flags)
+    ncell = int(np.prod(np.floor(sysm.dim / sysm.L)))
     eng.setdt(dt)
     eng.step()
-    t0 = time.perf_counter()
-    for _ in range(host_steps):
-        eng.step()
-    host_ms = (time.perf_counter() - t0) / host_steps * 1e3
+    host_ms = None
+    if host_steps > 0:
+        t0 = time.perf_counter()
+        for _ in range(host_steps):
+            eng.step()
+        host_ms = (time.perf_counter() - t0) / host_steps * 1e3
     eng.gpu_steps(3)
     t0 = time.perf_counter()
     eng.gpu_steps(10)
@@ -364,18 +381,32 @@ def md_step_section(nsteps, host_steps=5, dt=0.002):
     t0 = time.perf_counter()
     eng.gpu_steps(nsteps)
     t_long = time.perf_counter() - t0
+    rebuilds = None
+    if verlet:
+        # rebuild steps of one more call, counted step by step
+        rebuilds = 0
+        for _ in range(20):
+            eng.gpu_steps(1)
+            rebuilds += eng.maxdx(ncell)[0] == 0.0
     eng.close()
     res_ms = t_long / nsteps * 1e3
     # two call lengths separate the per-step cost from the per-call upload/download
     step_ms = (t_long - t_short) / (nsteps - 10) * 1e3 if nsteps > 10 else res_ms
     call_ms = t_short * 1e3 - 10 * step_ms
-    return {"system": "argon_1m: 1,048,576 LJ atoms, fcc 64^3 at 100 K, rc 1.0 nm, 1.0 nm cells",
-            "dt_ps": dt, "steps": nsteps,
-            "host_driven_ms_per_step": round(host_ms, 3), "resident_ms_per_step": round(res_ms, 3),
-            "resident_ms_per_step_marginal": round(step_ms, 3), "resident_ms_per_call": round(call_ms, 1),
-            "ns_per_day_host_driven": dt * 1e-3 * 86400.0 / (host_ms * 1e-3),
-            "ns_per_day_resident": dt * 1e-3 * 86400.0 / (res_ms * 1e-3),
-            "ns_per_day_resident_marginal": dt * 1e-3 * 86400.0 / (step_ms * 1e-3)}
+    h = sysm.dim[0] / np.floor(sysm.dim[0] / cell)
+    out = {"system": f"argon_1m: 1,048,576 LJ atoms, fcc 64^3 at 100 K, rc 1.0 nm, {ncell} cells of {h:.3f} nm",
+           "dt_ps": dt, "steps": nsteps, "pseudo_verlet": verlet,
+           "resident_ms_per_step": round(res_ms, 3),
+           "resident_ms_per_step_marginal": round(step_ms, 3), "resident_ms_per_call": round(call_ms, 1),
+           "ns_per_day_resident": dt * 1e-3 * 86400.0 / (res_ms * 1e-3),
+           "ns_per_day_resident_marginal": dt * 1e-3 * 86400.0 / (step_ms * 1e-3)}
+    if host_ms is not None:
+        out["host_driven_ms_per_step"] = round(host_ms, 3)
+        out["ns_per_day_host_driven"] = dt * 1e-3 * 86400.0 / (host_ms * 1e-3)
+    if verlet:
+        out["skin_nm"] = round(h - sysm.cutoff, 4)
+        out["rebuilds_in_20_steps"] = int(rebuilds)
+    return out
 
 
 def copy_calibration(mib):
