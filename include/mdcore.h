@@ -363,6 +363,41 @@ struct exclusion {
 extern int engine_err;
 extern const char *engine_err_msg[];
 
+/* ---------------------------------------------------------------- rigid */
+
+struct engine;
+
+/* Rigid constraint groups (rigid.h): up to rigid_maxparts particles and
+   rigid_maxconstr distance constraints, solved by SHAKE or P-SHAKE after
+   each advance. */
+#define rigid_err_ok          0
+#define rigid_err_null        -1
+#define rigid_err_malloc      -2
+#define rigid_maxparts        10
+#define rigid_maxconstr       (3 * rigid_maxparts)
+#define rigid_maxiter         100
+#define rigid_pshake_refine   10
+#define rigid_pshake_maxalpha 0.1f
+
+struct rigid {
+    int nr_parts;
+    int parts[rigid_maxparts]; /* particle ids */
+    int nr_constr;
+    struct {
+        int i, j; /* indices into parts */
+        double d2;
+    } constr[rigid_maxconstr];
+    /* P-SHAKE constraint coupling matrix, nr_constr x nr_constr used. */
+    float a[rigid_maxconstr * rigid_maxconstr];
+};
+
+/* P-SHAKE refines its coupling matrices on the first engine_pshake_steps
+   steps (engine.h:81). */
+#define engine_pshake_steps 20
+
+int rigid_eval_shake(struct rigid *rs, int N, struct engine *e);
+int rigid_eval_pshake(struct rigid *rs, int N, struct engine *e, int a_update);
+
 struct engine {
     unsigned int flags;
     struct space s;
@@ -380,6 +415,11 @@ struct engine {
     void *gpu;
     /* Wall-clock seconds of the last engine_step phases (new). */
     double timers[8];
+    /* Rigid constraints (engine.h:239-250). */
+    struct rigid *rigids;
+    int nr_rigids, rigids_size, nr_constr;
+    int *part2rigid, part2rigid_size;
+    double tol_rigid;
 };
 
 enum {
@@ -407,6 +447,15 @@ int engine_nonbond_eval(struct engine *e);
 int engine_advance(struct engine *e);
 /* Pseudo-Verlet bookkeeping at the start of a step (engine.c:193-315). */
 int engine_verlet_update(struct engine *e);
+/* Rigid constraints (engine_rigid.c:78-230, 352-583): add a distance
+   constraint d between particles pid and pjd (merging groups), and solve
+   all groups (SHAKE with engine_flag_shake, P-SHAKE otherwise). */
+int engine_rigid_add(struct engine *e, int pid, int pjd, double d);
+int engine_rigid_eval(struct engine *e);
+/* New helper: group rid's particle ids (parts[0] = count, then the ids),
+   constraint index pairs (ij, 2 per constraint) and d^2; returns the
+   number of constraints (rid == -1: the number of groups). */
+int engine_getrigid(struct engine *e, int rid, int *parts, int *ij, double *d2);
 /* New helper: s->maxdx and the per-cell maxdx (cell_maxdx may be NULL). */
 int engine_getmaxdx(struct engine *e, double *maxdx, double *cell_maxdx);
 /* New helper: by particle id, the cell-relative position (x, 3 per id) and

@@ -323,9 +323,10 @@ __global__ __launch_bounds__(128) void k_arrive(struct step_geom G, const int *_
 
 /* Pseudo-Verlet advance, one wave per cell: with kick, v += f dt / m and
    x += dt v in place (engine_advance's Verlet branch, engine.c:1961-1985:
-   no migration) and the cell's kinetic energy; then the largest squared
-   displacement from the rebuild positions, ((0 + w0^2) + w1^2) + w2^2 with
-   w = x - oldx as in engine_verlet_update (engine.c:222-233). */
+   no migration).  The cell's kinetic energy of the (new) velocities, and
+   with oldx the largest squared displacement from the rebuild positions,
+   ((0 + w0^2) + w1^2) + w2^2 with w = x - oldx as in engine_verlet_update
+   (engine.c:222-233). */
 __global__ __launch_bounds__(64) void k_advance_verlet(int kick, double dt, const double *__restrict__ mass,
                                                        const double *__restrict__ imass,
                                                        const float4 *__restrict__ fhi, const float4 *__restrict__ flo,
@@ -354,13 +355,18 @@ __global__ __launch_bounds__(64) void k_advance_verlet(int kick, double dt, cons
                 x[3 * a + m] = xa[m];
             }
             ke += 0.5 * mass[t] * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+        } else {
+            const double m = mass[type[a]], v0 = v[3 * a], v1 = v[3 * a + 1], v2 = v[3 * a + 2];
+            ke += 0.5 * m * (v0 * v0 + v1 * v1 + v2 * v2);
         }
-        double dx = 0.0;
-        for (int m = 0; m < 3; m++) {
-            const double w = xa[m] - oldx[3 * a + m];
-            dx += w * w;
+        if (oldx) {
+            double dx = 0.0;
+            for (int m = 0; m < 3; m++) {
+                const double w = xa[m] - oldx[3 * a + m];
+                dx += w * w;
+            }
+            lm = fmax(dx, lm);
         }
-        lm = fmax(dx, lm);
     }
     for (int s = 32; s > 0; s >>= 1) {
         ke += __shfl_xor(ke, s, 64);
@@ -427,20 +433,206 @@ __global__ __launch_bounds__(256) void k_cell_maxdx(int ncells, const double *__
     if (c < ncells) cell_maxdx[c] = sqrt(lmax2[c]);
 }
 
+/* ------------------------------------------------------------ rigid */
+
+#define MD_RIG_NP 4 /* particles per group on the device */
+#define MD_RIG_NC 6 /* constraints per group on the device */
+
+struct rigid_dev {
+    int n;                  /* groups */
+    const int *np, *nc;     /* per group */
+    const int *parts;       /* [n][MD_RIG_NP] particle ids */
+    const int *ij;          /* [n][MD_RIG_NC][2] indices into parts */
+    const double *d2;       /* [n][MD_RIG_NC] */
+    float *a;               /* [n][MD_RIG_NC^2] P-SHAKE coupling, kept */
+};
+
+/* Slot of each particle id and cell of each slot, current layout. */
+__global__ __launch_bounds__(256) void k_layout(int n, int ncells, const int *__restrict__ id,
+                                                const int *__restrict__ offset, int *__restrict__ slot_of_id,
+                                                int *__restrict__ cell_of) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) slot_of_id[id[a]] = a;
+    if (a < ncells)
+        for (int k = offset[a]; k < offset[a + 1]; k++) cell_of[k] = a;
+}
+
+/* One thread per rigid group: rigid_eval_shake (shake != 0, rigid.c:81-277)
+   or rigid_eval_pshake (rigid.c:279-523) with the host's FP64 (and, for the
+   coupling matrix, FP32) arithmetic in the same order.  fcorr[3 id + j]
+   receives P-SHAKE's force m (x_new - x) / dt^2; st[1] is set when a
+   corrected position left its cell. */
+__global__ __launch_bounds__(128) void k_rigid(struct rigid_dev R, int shake, int a_update, double dt, double tol,
+                                               struct step_geom G, const double *__restrict__ mass,
+                                               const int *__restrict__ slot_of_id, const int *__restrict__ cell_of,
+                                               const int *__restrict__ type, double *__restrict__ x,
+                                               double *__restrict__ v, double *__restrict__ fcorr,
+                                               int *__restrict__ st) {
+#pragma clang fp contract(off)
+    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rid >= R.n) return;
+    const int np = R.np[rid], nc = R.nc[rid];
+    int slot[MD_RIG_NP], cell[MD_RIG_NP], shift[3 * MD_RIG_NP], ci[MD_RIG_NC], cj[MD_RIG_NC];
+    double m[MD_RIG_NP], xp[3 * MD_RIG_NP], xo[3 * MD_RIG_NP], res[MD_RIG_NC], d2[MD_RIG_NC];
+    const double idt = 1.0 / dt;
+    for (int k = 0; k < nc; k++) {
+        ci[k] = R.ij[(rid * MD_RIG_NC + k) * 2];
+        cj[k] = R.ij[(rid * MD_RIG_NC + k) * 2 + 1];
+        d2[k] = R.d2[rid * MD_RIG_NC + k];
+    }
+    for (int k = 0; k < np; k++)
+        if (slot_of_id[R.parts[rid * MD_RIG_NP + k]] < 0) return; /* not loaded: skipped, as on the host */
+    for (int k = 0; k < np; k++) {
+        const int a = slot_of_id[R.parts[rid * MD_RIG_NP + k]];
+        slot[k] = a;
+        cell[k] = cell_of[a];
+        m[k] = mass[type[a]];
+    }
+    const int c0 = cell[0];
+    const int l0[3] = {c0 / (G.cdim[1] * G.cdim[2]), (c0 / G.cdim[2]) % G.cdim[1], c0 % G.cdim[2]};
+    for (int k = 0; k < np; k++) {
+        const int ck = cell[k];
+        const int lk[3] = {ck / (G.cdim[1] * G.cdim[2]), (ck / G.cdim[2]) % G.cdim[1], ck % G.cdim[2]};
+        for (int j = 0; j < 3; j++) {
+            int sh = 0;
+            if (ck != c0) {
+                sh = lk[j] - l0[j];
+                if (sh > 1)
+                    sh = -1;
+                else if (sh < -1)
+                    sh = 1;
+            }
+            shift[3 * k + j] = sh;
+            const double xk = x[3 * slot[k] + j];
+            xp[3 * k + j] = (ck != c0) ? xk + G.h[j] * sh : xk;
+            xo[3 * k + j] = xp[3 * k + j] - dt * v[3 * slot[k] + j];
+        }
+    }
+    if (shake) {
+        double vc[3 * MD_RIG_NC], wvc[3 * MD_RIG_NC];
+        for (int k = 0; k < nc; k++)
+            for (int j = 0; j < 3; j++) {
+                vc[3 * k + j] = xo[3 * ci[k] + j] - xo[3 * cj[k] + j];
+                wvc[3 * k + j] = 1.0 / (m[ci[k]] + m[cj[k]]) * vc[3 * k + j];
+            }
+        for (int iter = 0; iter < rigid_maxiter; iter++) {
+            double max_res = 0.0;
+            for (int k = 0; k < nc; k++) {
+                const int pid = ci[k], pjd = cj[k];
+                res[k] = d2[k];
+                for (int j = 0; j < 3; j++) res[k] -= (xp[3 * pid + j] - xp[3 * pjd + j]) * (xp[3 * pid + j] - xp[3 * pjd + j]);
+                if (fabs(res[k]) > max_res) max_res = fabs(res[k]);
+            }
+            if (max_res < tol) break;
+            for (int k = 0; k < nc; k++) {
+                const int pid = ci[k], pjd = cj[k];
+                const double lambda =
+                    0.5 * res[k] /
+                    ((xp[3 * pid] - xp[3 * pjd]) * vc[3 * k] + (xp[3 * pid + 1] - xp[3 * pjd + 1]) * vc[3 * k + 1] +
+                     (xp[3 * pid + 2] - xp[3 * pjd + 2]) * vc[3 * k + 2]);
+                for (int j = 0; j < 3; j++) {
+                    const double w = lambda * wvc[3 * k + j];
+                    xp[3 * pid + j] += w * m[pjd];
+                    xp[3 * pjd + j] -= w * m[pid];
+                }
+            }
+        }
+    } else {
+        float *A = R.a + rid * MD_RIG_NC * MD_RIG_NC;
+        double vc[3 * MD_RIG_NC * MD_RIG_NP];
+        for (int q = 0; q < 3 * nc * np; q++) vc[q] = 0.0;
+        for (int k = 0; k < nc; k++) {
+            const int pid = ci[k], pjd = cj[k];
+            double dv[3];
+            for (int j = 0; j < 3; j++) dv[j] = (xo[3 * pid + j] - xo[3 * pjd + j]) / (m[pid] + m[pjd]);
+            for (int i = 0; i < nc; i++)
+                if (A[k * nc + i] != 0.0f)
+                    for (int j = 0; j < 3; j++) {
+                        vc[i * np * 3 + pid * 3 + j] += A[k * nc + i] * dv[j] * m[pjd];
+                        vc[i * np * 3 + pjd * 3 + j] += -A[k * nc + i] * dv[j] * m[pid];
+                    }
+        }
+        int iter;
+        for (iter = 0; iter < rigid_maxiter; iter++) {
+            double max_res = 0.0;
+            for (int k = 0; k < nc; k++) {
+                const double *g = &vc[k * 3 * np];
+                const int pid = ci[k], pjd = cj[k];
+                res[k] = d2[k];
+                for (int j = 0; j < 3; j++) res[k] -= (xp[3 * pid + j] - xp[3 * pjd + j]) * (xp[3 * pid + j] - xp[3 * pjd + j]);
+                if (fabs(res[k]) > max_res) max_res = fabs(res[k]);
+                const double lambda = 0.5 * res[k] /
+                                      ((xp[3 * pid] - xp[3 * pjd]) * (g[3 * pid] - g[3 * pjd]) +
+                                       (xp[3 * pid + 1] - xp[3 * pjd + 1]) * (g[3 * pid + 1] - g[3 * pjd + 1]) +
+                                       (xp[3 * pid + 2] - xp[3 * pjd + 2]) * (g[3 * pid + 2] - g[3 * pjd + 2]));
+                for (int j = 0; j < 3 * np; j++) xp[j] += lambda * g[j];
+            }
+            if (max_res < tol) break;
+        }
+        if (nc > 1 && (a_update || iter > rigid_pshake_refine)) {
+            float a_new[MD_RIG_NC * MD_RIG_NC], tmp[MD_RIG_NC * MD_RIG_NC];
+            float w, dx[3], max_alpha = 0.0f;
+            for (int i = 0; i < nc; i++) {
+                const double *gi = &vc[i * 3 * np];
+                const int pid = ci[i], pjd = cj[i];
+                for (int j = 0; j < 3; j++) dx[j] = xp[3 * pid + j] - xp[3 * pjd + j];
+                w = -1.0 / (dx[0] * (gi[pid * 3] - gi[pjd * 3]) + dx[1] * (gi[pid * 3 + 1] - gi[pjd * 3 + 1]) +
+                            dx[2] * (gi[pid * 3 + 2] - gi[pjd * 3 + 2]));
+                for (int j = i + 1; j < nc; j++) {
+                    const double *gj = &vc[j * 3 * np];
+                    a_new[i + j * nc] = w * (dx[0] * (gj[pid * 3] - gj[pjd * 3]) + dx[1] * (gj[pid * 3 + 1] - gj[pjd * 3 + 1]) +
+                                             dx[2] * (gj[pid * 3 + 2] - gj[pjd * 3 + 2]));
+                    a_new[j + i * nc] = a_new[i + j * nc];
+                    max_alpha = fmaxf(max_alpha, fabsf(a_new[i + j * nc]));
+                }
+                a_new[i + i * nc] = 1.0f;
+            }
+            if (max_alpha > rigid_pshake_maxalpha) {
+                w = 0.1 / max_alpha;
+                for (int i = 0; i < nc * nc; i++) a_new[i] *= w;
+                for (int i = 0; i < nc; i++) a_new[i * nc + i] = 1.0f;
+            }
+            for (int i = 0; i < nc * nc; i++) tmp[i] = 0.0f;
+            for (int j = 0; j < nc; j++)
+                for (int k = 0; k < nc; k++)
+                    for (int i = 0; i < nc; i++) tmp[j * nc + i] += A[k * nc + i] * a_new[j * nc + k];
+            for (int i = 0; i < nc * nc; i++) A[i] = tmp[i];
+        }
+    }
+    int outside = 0;
+    for (int k = 0; k < np; k++) {
+        const int a = slot[k], id = R.parts[rid * MD_RIG_NP + k];
+        for (int j = 0; j < 3; j++) {
+            const double xn = (cell[k] != c0) ? xp[3 * k + j] - G.h[j] * shift[3 * k + j] : xp[3 * k + j];
+            if (!shake) fcorr[3 * id + j] = idt * idt * (xn - x[3 * a + j]) * m[k];
+            v[3 * a + j] += idt * (xn - x[3 * a + j]);
+            x[3 * a + j] = xn;
+            outside |= (xn >= G.h[j]) | (xn < 0.0);
+        }
+    }
+    if (outside) st[1] = 1;
+}
+
 /* ------------------------------------------------------------ host side */
 
 struct step_bufs {
     double *v, *x2, *v2, *mass, *imass, *cell_ke, *bke, *esteps;
     double *oldx, *lmax2, *bmx, *vmax; /* pseudo-Verlet */
+    /* rigid groups: device copies, slot of each id, P-SHAKE forces by id */
+    int *rg_np, *rg_nc, *rg_parts, *rg_ij, *sid;
+    double *rg_d2, *fcorr;
+    float *rg_a;
+    int nrig;
     int *id2, *type2, *slot, *qsrc, *qdest, *stay, *ndep, *narr, *bsum, *bmax, *st, *count2, *offset2;
     int nblocks;
 };
 
 static void step_free(struct step_bufs *b) {
     double *d[] = {b->v, b->x2, b->v2, b->mass, b->imass, b->cell_ke, b->bke, b->esteps,
-                   b->oldx, b->lmax2, b->bmx, b->vmax};
-    int *i[] = {b->id2, b->type2, b->slot, b->qsrc, b->qdest, b->stay, b->ndep,
-                b->narr, b->bsum, b->bmax, b->st, b->count2, b->offset2};
+                   b->oldx, b->lmax2, b->bmx, b->vmax, b->rg_d2, b->fcorr};
+    int *i[] = {b->id2, b->type2, b->slot, b->qsrc, b->qdest, b->stay, b->ndep, b->narr, b->bsum,
+                b->bmax, b->st, b->count2, b->offset2, b->rg_np, b->rg_nc, b->rg_parts, b->rg_ij, b->sid};
+    (void)hipFree(b->rg_a);
     for (size_t k = 0; k < sizeof(d) / sizeof(d[0]); k++) (void)hipFree(d[k]);
     for (size_t k = 0; k < sizeof(i) / sizeof(i[0]); k++) (void)hipFree(i[k]);
     memset(b, 0, sizeof(*b));
@@ -600,6 +792,36 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
             c->parts[k].f[2] += (double)g->h_fhi[a].z + (double)g->h_flo[a].z;
         }
     }
+    if (b->nrig > 0) {
+        /* P-SHAKE's coupling matrices and the constraint forces of the last
+           step, added as rigid_eval_pshake adds them */
+        const int shake = (e->flags & engine_flag_shake) != 0;
+        float *ra = (float *)malloc(sizeof(float) * (size_t)b->nrig * MD_RIG_NC * MD_RIG_NC);
+        double *fc = shake ? NULL : (double *)malloc(sizeof(double) * 3 * (size_t)g->nids);
+        if (ra == NULL || (!shake && fc == NULL) ||
+            hipMemcpy(ra, b->rg_a, sizeof(float) * (size_t)b->nrig * MD_RIG_NC * MD_RIG_NC, hipMemcpyDeviceToHost) !=
+                hipSuccess ||
+            (!shake && hipMemcpy(fc, b->fcorr, sizeof(double) * 3 * g->nids, hipMemcpyDeviceToHost) != hipSuccess)) {
+            free(ra);
+            free(fc);
+            rc = gerror_(engine_err_gpu, "rigid download failed");
+            goto done;
+        }
+        for (int r = 0; r < b->nrig; r++)
+            memcpy(e->rigids[r].a, &ra[(size_t)r * MD_RIG_NC * MD_RIG_NC],
+                   sizeof(float) * e->rigids[r].nr_constr * e->rigids[r].nr_constr);
+        if (!shake)
+            for (int cid = 0; cid < nc; cid++) {
+                struct cell *c = &s->cells[cid];
+                for (int k = 0; k < c->count; k++) {
+                    const int id = c->parts[k].id;
+                    if (id < e->part2rigid_size && e->part2rigid[id] >= 0)
+                        for (int j = 0; j < 3; j++) c->parts[k].f[j] += fc[3 * id + j];
+                }
+            }
+        free(ra);
+        free(fc);
+    }
     s->epot_nonbond += g->h_tot[0];
     s->epot_exclusion += g->h_tot[1];
     s->epot += g->h_tot[0] + g->h_tot[1];
@@ -621,6 +843,97 @@ done:
     free(count);
     free(ioi);
     return rc;
+}
+
+/* Rigid groups to the device (engine_gpu_steps handles groups of up to
+   MD_RIG_NP particles and MD_RIG_NC constraints, e.g. rigid water). */
+static int rigid_upload(struct engine *e, struct gpu_state *g, struct step_bufs *b) {
+    const int n = e->nr_rigids;
+    int rc = engine_err_ok;
+    b->nrig = n;
+    if (n == 0) return engine_err_ok;
+    for (int r = 0; r < n; r++) {
+        if (e->rigids[r].nr_parts > MD_RIG_NP || e->rigids[r].nr_constr > MD_RIG_NC)
+            return gerror_(engine_err_range, "rigid groups of more than 4 particles or 6 constraints run through engine_step");
+        for (int k = 0; k < e->rigids[r].nr_parts; k++)
+            if (e->rigids[r].parts[k] < 0 || e->rigids[r].parts[k] >= g->nids)
+                return gerror_(engine_err_range, "rigid constraint on a particle id unknown to the device");
+    }
+    int *np = (int *)calloc(n, sizeof(int)), *nc = (int *)calloc(n, sizeof(int));
+    int *parts = (int *)calloc((size_t)n * MD_RIG_NP, sizeof(int));
+    int *ij = (int *)calloc((size_t)n * MD_RIG_NC * 2, sizeof(int));
+    double *d2 = (double *)calloc((size_t)n * MD_RIG_NC, sizeof(double));
+    float *a = (float *)calloc((size_t)n * MD_RIG_NC * MD_RIG_NC, sizeof(float));
+    if (np == NULL || nc == NULL || parts == NULL || ij == NULL || d2 == NULL || a == NULL) {
+        rc = gerror_(engine_err_malloc, "malloc failed");
+        goto done;
+    }
+    for (int r = 0; r < n; r++) {
+        const struct rigid *R = &e->rigids[r];
+        np[r] = R->nr_parts;
+        nc[r] = R->nr_constr;
+        for (int k = 0; k < R->nr_parts; k++) parts[r * MD_RIG_NP + k] = R->parts[k];
+        for (int k = 0; k < R->nr_constr; k++) {
+            ij[(r * MD_RIG_NC + k) * 2] = R->constr[k].i;
+            ij[(r * MD_RIG_NC + k) * 2 + 1] = R->constr[k].j;
+            d2[r * MD_RIG_NC + k] = R->constr[k].d2;
+        }
+        memcpy(&a[(size_t)r * MD_RIG_NC * MD_RIG_NC], R->a, sizeof(float) * R->nr_constr * R->nr_constr);
+    }
+    if (hipMalloc(&b->rg_np, sizeof(int) * n) != hipSuccess || hipMalloc(&b->rg_nc, sizeof(int) * n) != hipSuccess ||
+        hipMalloc(&b->rg_parts, sizeof(int) * n * MD_RIG_NP) != hipSuccess ||
+        hipMalloc(&b->rg_ij, sizeof(int) * n * MD_RIG_NC * 2) != hipSuccess ||
+        hipMalloc(&b->rg_d2, sizeof(double) * n * MD_RIG_NC) != hipSuccess ||
+        hipMalloc(&b->rg_a, sizeof(float) * n * MD_RIG_NC * MD_RIG_NC) != hipSuccess ||
+        hipMalloc(&b->sid, sizeof(int) * g->nids) != hipSuccess ||
+        hipMalloc(&b->fcorr, sizeof(double) * 3 * g->nids) != hipSuccess ||
+        hipMemcpy(b->rg_np, np, sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->rg_nc, nc, sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->rg_parts, parts, sizeof(int) * n * MD_RIG_NP, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->rg_ij, ij, sizeof(int) * n * MD_RIG_NC * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->rg_d2, d2, sizeof(double) * n * MD_RIG_NC, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->rg_a, a, sizeof(float) * n * MD_RIG_NC * MD_RIG_NC, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(b->fcorr, 0, sizeof(double) * 3 * g->nids) != hipSuccess ||
+        hipMemset(b->sid, 0xff, sizeof(int) * g->nids) != hipSuccess)
+        rc = gerror_(engine_err_gpu, "rigid upload failed");
+done:
+    free(np);
+    free(nc);
+    free(parts);
+    free(ij);
+    free(d2);
+    free(a);
+    return rc;
+}
+
+/* After an advance: the constraints (engine.c:2196-2232), then the kinetic
+   energy of the corrected velocities and, with oldx, the displacements, and
+   the step's energies into es. */
+static int rigid_step(struct engine *e, struct gpu_state *g, struct step_bufs *b, const struct step_geom &G,
+                      const double *oldx, double *es) {
+    const int n = g->nparts, nc = G.ncells;
+    const int nbl = ((n > nc ? n : nc) + 255) / 256;
+    struct rigid_dev R;
+    R.n = b->nrig;
+    R.np = b->rg_np;
+    R.nc = b->rg_nc;
+    R.parts = b->rg_parts;
+    R.ij = b->rg_ij;
+    R.d2 = b->rg_d2;
+    R.a = b->rg_a;
+    hipLaunchKernelGGL(k_layout, dim3(nbl > 0 ? nbl : 1), dim3(256), 0, g->stream, n, nc, g->d_id, g->d_offset, b->sid,
+                       g->d_cell_of);
+    hipLaunchKernelGGL(k_rigid, dim3((R.n + 127) / 128), dim3(128), 0, g->stream, R,
+                       (e->flags & engine_flag_shake) != 0, (e->time + 1) < engine_pshake_steps, e->dt, e->tol_rigid,
+                       G, b->mass, b->sid, g->d_cell_of, g->d_type, g->d_x, b->v, b->fcorr, b->st);
+    hipLaunchKernelGGL(k_advance_verlet, dim3(nc), dim3(64), 0, g->stream, 0, 0.0, b->mass, b->imass, g->d_fhi,
+                       g->d_flo, g->d_count, g->d_offset, g->d_x, b->v, g->d_type, oldx, b->lmax2, b->cell_ke);
+    hipLaunchKernelGGL(k_verlet_red_a, dim3(b->nblocks), dim3(256), 0, g->stream, nc, b->cell_ke, b->lmax2, b->bke,
+                       b->bmx);
+    hipLaunchKernelGGL(k_verlet_red_b, dim3(1), dim3(64), 0, g->stream, b->nblocks, b->bke, b->bmx, g->d_tot, es,
+                       b->vmax);
+    if (hipGetLastError() != hipSuccess) return gerror_(engine_err_gpu, "constraints failed");
+    return engine_err_ok;
 }
 
 /* Largest squared displacement of the last k_advance_verlet. */
@@ -708,6 +1021,7 @@ static int steps_verlet(struct engine *e, struct gpu_state *g, struct step_bufs 
             rc = gerror_(engine_err_gpu, "advance failed");
             break;
         }
+        if (b->nrig > 0 && (rc = rigid_step(e, g, b, G, b->oldx, b->esteps + 4 * step)) < 0) break;
         if ((rc = verlet_status(g, b, &vmax)) < 0) break;
         e->time += 1;
     }
@@ -762,6 +1076,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     }
     const int verlet = (e->flags & engine_flag_verlet) != 0;
     double smaxdx = s->maxdx;
+    if (rc == engine_err_ok) rc = rigid_upload(e, g, &b);
     if (verlet && rc == engine_err_ok)
         rc = steps_verlet(e, g, &b, G, nsteps, &smaxdx);
     /* engine_step: shuffle, pass, advance.  After an advance that left every
@@ -776,6 +1091,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
         g->P.maxn = maxc > 0 ? maxc : 1;
         if ((rc = gpu_launch(g, 0, (e->flags & engine_flag_mlp) != 0, 1)) < 0) break;
         if ((rc = step_migrate(g, &b, G, e->dt, b.esteps + 4 * step)) < 0 ||
+            (b.nrig > 0 && (rc = rigid_step(e, g, &b, G, NULL, b.esteps + 4 * step)) < 0) ||
             (rc = step_status(g, &b, &maxc, &outside)) < 0)
             break;
         e->time += 1;

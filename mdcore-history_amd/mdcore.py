@@ -27,6 +27,7 @@ engine_flag_cuda = 4
 engine_flag_nullpart = 32768
 engine_flag_projwindow = 65536
 engine_flag_verlet = 16
+engine_flag_shake = 1024
 engine_flag_verlet_pseudo = 256
 engine_flag_mlp = 131072
 
@@ -86,6 +87,9 @@ def library():
         "engine_getlocal": (C.c_int, [C.c_void_p, _dp, _ip, C.c_int]),
         "engine_getmaxdx": (C.c_int, [C.c_void_p, _dp, _dp]),
         "engine_verlet_update": (C.c_int, [C.c_void_p]),
+        "engine_rigid_add": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
+        "engine_rigid_eval": (C.c_int, [C.c_void_p]),
+        "engine_getrigid": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp]),
         "engine_advance": (C.c_int, [C.c_void_p]),
         "engine_gettasks": (C.c_int, [C.c_void_p, _ip, C.c_int]),
         "engine_prepare": (C.c_int, [C.c_void_p]),
@@ -352,6 +356,29 @@ class Engine:
         cm = np.zeros(max(ncells, 1))
         rb = check(self.lib.engine_getmaxdx(self.e, ptr(m), ptr(cm)), "engine_getmaxdx")
         return float(m[0]), cm[:ncells], rb
+
+    def rigid_add(self, i, j, d):
+        check(self.lib.engine_rigid_add(self.e, int(i), int(j), float(d)), "engine_rigid_add")
+
+    def rigids_add(self, triples):
+        """(i, j, d) distance constraints, in order."""
+        for i, j, d in triples:
+            self.rigid_add(i, j, d)
+
+    def rigid_eval(self):
+        check(self.lib.engine_rigid_eval(self.e), "engine_rigid_eval")
+
+    def nrigids(self):
+        return check(self.lib.engine_getrigid(self.e, -1, None, None, None), "engine_getrigid")
+
+    def rigid(self, rid):
+        """(particle ids, constraint index pairs, d^2) of group rid."""
+        parts = np.zeros(1 + 10, dtype=np.int32)
+        ij = np.zeros(60, dtype=np.int32)
+        d2 = np.zeros(30)
+        nc = check(self.lib.engine_getrigid(self.e, rid, ptr(parts, C.c_int), ptr(ij, C.c_int), ptr(d2)),
+                   "engine_getrigid")
+        return parts[1:1 + parts[0]].copy(), ij[:2 * nc].reshape(-1, 2).copy(), d2[:nc].copy()
 
     def verlet_update(self):
         check(self.lib.engine_verlet_update(self.e), "engine_verlet_update")

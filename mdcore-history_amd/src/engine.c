@@ -103,6 +103,7 @@ int engine_init(struct engine *e, const double *origin, const double *dim, doubl
     e->flags = flags;
     e->time = 0;
     e->dt = 0.0;
+    e->tol_rigid = 1e-6;
     e->exclusions_size = 100;
     if ((e->exclusions = (struct exclusion *)malloc(sizeof(struct exclusion) * e->exclusions_size)) == NULL)
         return eerror_(engine_err_malloc);
@@ -394,6 +395,8 @@ int engine_step(struct engine *e) {
     if ((e->flags & engine_flag_verlet) && e->s.verlet_rebuild) e->s.verlet_rebuild = 0;
     t = wall_seconds();
     if (engine_advance(e) < 0) return eerror_(engine_err);
+    /* constraints after the advance (engine.c:2196-2232) */
+    if (e->nr_rigids > 0 && engine_rigid_eval(e) < 0) return eerror_(engine_err_rigid);
     e->timers[engine_timer_advance] = wall_seconds() - t;
     e->timers[engine_timer_step] = wall_seconds() - t0;
     return engine_err_ok;
@@ -446,6 +449,11 @@ int engine_finalize(struct engine *e) {
     free(e->types);
     free(e->p);
     free(e->exclusions);
+    free(e->rigids);
+    free(e->part2rigid);
+    e->rigids = NULL;
+    e->part2rigid = NULL;
+    e->nr_rigids = e->rigids_size = e->part2rigid_size = 0;
     e->types = NULL;
     e->p = NULL;
     e->exclusions = NULL;

@@ -41,6 +41,7 @@ class System:
     excl: np.ndarray = None    # (M, 2) particle id pairs
     v: np.ndarray = None
     meta: dict = field(default_factory=dict)
+    rigid: list = None         # (i, j, d) distance constraints
 
     @property
     def N(self):
@@ -69,7 +70,9 @@ def argon(n=20, a=0.565, seed=6178, T=100.0, tol=1e-3, jitter=0.0):
 
 
 def water(nmol=33333, box=10.0, lattice=(33, 33, 31), seed=1234, jitter=0.02, a=0.05, eps_rf=0.0, tol=1e-3,
-          cutoff=1.0, cell=1.0):
+          cutoff=1.0, cell=1.0, rigid=False):
+    """rigid: O-H1, O-H2 (0.1 nm) and H1-H2 distance constraints per
+    molecule, in that order (SPC/E geometry)."""
     qO, qH = -0.8476, 0.4238
     A, B = 2.6378e-6, 2.6192e-3
     x = md.gen_water(nmol, [box] * 3, lattice, 0.1, math.radians(109.47), jitter, seed)
@@ -80,9 +83,14 @@ def water(nmol=33333, box=10.0, lattice=(33, 33, 31), seed=1234, jitter=0.02, a=
     pHH = md.LJ126_RF(a, cutoff, 0.0, 0.0, qH * qH, KE, eps_rf, tol)
     m = np.arange(nmol, dtype=np.int64) * 3
     excl = np.concatenate([np.stack([m, m + 1], 1), np.stack([m, m + 2], 1), np.stack([m + 1, m + 2], 1)])
+    cons = None
+    if rigid:
+        dhh = 2 * 0.1 * math.sin(math.radians(109.47) / 2)
+        cons = [c for k in range(nmol) for c in ((3 * k, 3 * k + 1, 0.1), (3 * k, 3 * k + 2, 0.1),
+                                                (3 * k + 1, 3 * k + 2, dhh))]
     return System("water", np.zeros(3), np.full(3, box), np.full(3, cell), cutoff, x, types,
                   np.array([0.0, 15.9994, 1.008]), [(1, 1, pOO), (1, 2, pOH), (2, 2, pHH)], excl, None,
-                  {"qO": qO, "qH": qH, "box": box, "nmol": nmol})
+                  {"qO": qO, "qH": qH, "box": box, "nmol": nmol}, cons)
 
 
 def mixture_species(nspecies=20, seed=20):
@@ -140,6 +148,8 @@ def build_engine(sys, flags=md.engine_flag_gpu, start=True):
     if sys.excl is not None and len(sys.excl):
         e.exclusions_add(sys.excl)
         e.exclusion_shrink()
+    if sys.rigid:
+        e.rigids_add(sys.rigid)
     if start:
         e.start()
     return e
