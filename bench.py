@@ -20,7 +20,8 @@ passes timed with HIP events on the engine's stream.  Prints one JSON line.
                 error against FP64, and a table set rebuilt at the tolerance
                 whose force error matches the network's, timed the same way
   nve           energy drift of 1,000 NVE steps (dt 2 fs) of the 32,000-atom
-                argon system with the table pass and with a trained network
+                argon system with the table pass and with a trained network,
+                and of the 100k-atom rigid water on the device-resident step
   md_step       whole MD steps (shuffle, pass, kick-drift, migration) of the
                 1,048,576-atom argon crystal at 100 K, dt 2 fs: host-driven
                 engine_step (positions up, forces down, host integration)
@@ -334,8 +335,9 @@ def nve_section(steps=1000, dt=0.002, equil=1000):
     t2 = time.perf_counter()
     rt, at = sg.drift_per_ns(e_tab, dt)
     rm, am = sg.drift_per_ns(e_mlp, dt)
+    water = water_nve(steps, dt, equil)
     return {"system": "argon_32k: 32,000 LJ atoms, fcc start at 100 K, rc 1.0 nm", "steps": steps, "dt_ps": dt,
-            "equilibration_steps": equil,
+            "equilibration_steps": equil, "water_rigid": water,
             "table": {"drift_rel_per_ns": rt, "drift_kJ_mol_per_ns": at, "e_mean": float(e_tab.mean()),
                       "e_std": float(e_tab.std()), "wall_s": round(t1 - t0, 1)},
             "mlp": {"drift_rel_per_ns": rm, "drift_kJ_mol_per_ns": am, "e_mean": float(e_mlp.mean()),
@@ -407,6 +409,42 @@ def md_step_run(nsteps, host_steps, dt, cell, verlet):
         out["skin_nm"] = round(h - sysm.cutoff, 4)
         out["rebuilds_in_20_steps"] = int(rebuilds)
     return out
+
+
+def water_nve(steps, dt, equil):
+    """Rigid water (P-SHAKE on the device) through engine_gpu_steps: 33,333
+    molecules, 0.5 nm cells with the projected window (the reference window
+    misses in-cutoff pairs on half-cutoff cells), Maxwell start at 300 K
+    from the lattice; energy drift after equilibration and the largest
+    constraint residue at the end."""
+    import dataclasses
+    import numpy as np
+    import mdcore as md
+    import surrogate as sg
+    import systems
+    sysm = systems.water(rigid=True, cell=0.5)
+    sysm = dataclasses.replace(sysm, v=md.gen_maxwell(sysm.masses[sysm.types], systems.KB * 300.0, 11))
+    eng = systems.build_engine(sysm, md.engine_flag_gpu | md.engine_flag_projwindow)
+    eng.setdt(dt)
+    t0 = time.perf_counter()
+    eng.gpu_steps(equil)
+    ep, ek = eng.gpu_steps(steps)
+    wall = time.perf_counter() - t0
+    x = eng.positions()
+    eng.close()
+    c = np.array(sysm.rigid)
+    i, j, d = c[:, 0].astype(int), c[:, 1].astype(int), c[:, 2]
+    dv = x[i] - x[j]
+    dv -= sysm.dim * np.round(dv / sysm.dim)
+    res = np.abs(d * d - (dv * dv).sum(1)).max()
+    et = ep + ek
+    rel, absd = sg.drift_per_ns(et, dt)
+    dof = 3 * sysm.N - len(c) - 3
+    return {"system": "water_100k rigid: 33,333 molecules (99,999 atoms), 10 nm box, LJ+RF, rc 1.0 nm, "
+                      "0.5 nm cells, projected window, P-SHAKE",
+            "drift_rel_per_ns": rel, "drift_kJ_mol_per_ns": absd, "e_mean": float(et.mean()), "e_std": float(et.std()),
+            "temperature_K": float(2 * ek.mean() / (dof * systems.KB)),
+            "max_constraint_residue_nm2": float(res), "ms_per_step": round(wall / (equil + steps) * 1e3, 3)}
 
 
 def copy_calibration(mib):

@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
     sort_desc(list, n, flag);
 }
 
-/* Sort by rank: one wave per cell, all 13 directions at once.  A key that no
+/* Sort by rank: one workgroup per cell, all 13 directions at once.  A key that no
    other particle of its list shares has exactly (number of greater keys)
    particles before it in any descending order, the reference's quicksort
    included, so those particles are placed by counting.  Only the order of
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
    MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions, 13 * maxn list
    words, 13 quicksort stacks. */
 #define MD_SORT_LDS_MAX 191
-__global__ __launch_bounds__(64) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
+__global__ __launch_bounds__(256) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
                                                   const int *__restrict__ offset,
                                                   const unsigned int *__restrict__ sortflags,
                                                   const double *__restrict__ x, unsigned int *__restrict__ sortlist,
@@ -331,9 +331,9 @@ __global__ __launch_bounds__(64) void k_sort_rank(struct nb_params P, const int 
     const unsigned int fl = sortflags[c];
     const int off = offset[c];
     if (lane < 13) tie[lane] = 0;
-    for (int k = lane; k < 3 * n; k += 64) xs[k] = x[3 * (size_t)off + k];
+    for (int k = lane; k < 3 * n; k += (int)blockDim.x) xs[k] = x[3 * (size_t)off + k];
     __syncthreads();
-    for (int t = lane; t < 13 * n; t += 64) {
+    for (int t = lane; t < 13 * n; t += (int)blockDim.x) {
         const int sid = t / n, i = t - sid * n;
         if (!((fl >> sid) & 1u)) continue;
         const unsigned int key = sort_key(&xs[3 * i], sid, P.dscale, P.bias);
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(64) void k_sort_rank(struct nb_params P, const int 
     }
     __syncthreads();
     unsigned int *out = sortlist + 13 * (size_t)off;
-    for (int t = lane; t < 13 * n; t += 64) {
+    for (int t = lane; t < 13 * n; t += (int)blockDim.x) {
         const int sid = t / n, i = t - sid * n;
         if (!((fl >> sid) & 1u)) continue;
         const unsigned int *L = lists + sid * n;
@@ -1581,7 +1581,9 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
         const int maxn = g->maxcount > 0 ? g->maxcount : 1;
         const size_t sh = sizeof(double) * 3 * maxn + sizeof(unsigned int) * 13 * maxn +
                           sizeof(int) * 13 * 2 * MD_SORT_STACK;
-        hipLaunchKernelGGL(k_sort_rank, dim3(g->ncells), dim3(64), sh, g->stream, P, g->d_count, g->d_offset,
+        /* one wave per cell; four for cells large enough to keep them busy */
+        const int bs = maxn > 48 ? 256 : 64;
+        hipLaunchKernelGGL(k_sort_rank, dim3(g->ncells), dim3(bs), sh, g->stream, P, g->d_count, g->d_offset,
                            g->d_sortflags, g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE, maxn);
     } else {
         nb = (g->ncells * 13 + 255) / 256;
