@@ -391,6 +391,26 @@ struct rigid {
     float a[rigid_maxconstr * rigid_maxconstr];
 };
 
+/* Bonded topology as the structure readers store it (bond.h, angle.h,
+   dihedral.h).  This library evaluates no bonded terms: the lists serve to
+   build exclusions and rigid constraints, and pid stays -1 (or -2 for an
+   improper dihedral, as engine_read_psf marks them). */
+struct bond {
+    int i, j, pid;
+};
+struct angle {
+    int i, j, k, pid;
+};
+struct dihedral {
+    int i, j, k, l, pid;
+};
+
+/* Inner radius of the tables made by the parameter readers: where the pair
+   potential reaches engine_maxKcutoff times its value at r0 (engine.h:82);
+   reader buffer size (engine.h:79). */
+#define engine_maxKcutoff 2
+#define engine_readbuff   16384
+
 /* P-SHAKE refines its coupling matrices on the first engine_pshake_steps
    steps (engine.h:81). */
 #define engine_pshake_steps 20
@@ -420,6 +440,13 @@ struct engine {
     int nr_rigids, rigids_size, nr_constr;
     int *part2rigid, part2rigid_size;
     double tol_rigid;
+    /* Bonded topology (engine.h:213-233). */
+    struct bond *bonds;
+    int nr_bonds, bonds_size;
+    struct angle *angles;
+    int nr_angles, angles_size;
+    struct dihedral *dihedrals;
+    int nr_dihedrals, dihedrals_size;
 };
 
 enum {
@@ -456,6 +483,40 @@ int engine_rigid_eval(struct engine *e);
    constraint index pairs (ij, 2 per constraint) and d^2; returns the
    number of constraints (rid == -1: the number of groups). */
 int engine_getrigid(struct engine *e, int rid, int *parts, int *ij, double *d2);
+
+/* Topology (engine_bonded.c): append a bond, angle or dihedral between
+   particle ids; pid is the (unused here) potential id. */
+int engine_bond_add(struct engine *e, int i, int j);
+int engine_angle_add(struct engine *e, int i, int j, int k, int pid);
+int engine_dihedral_add(struct engine *e, int i, int j, int k, int l, int pid);
+/* Type lookup by name (engine.c:1496-1540): the first type whose name
+   (engine_gettype) or second name (engine_gettype2) matches, or
+   engine_err_range. */
+int engine_gettype(struct engine *e, char *name);
+int engine_gettype2(struct engine *e, char *name2);
+
+/* Structure and parameter readers (engine_io.c), on open file descriptors:
+   - engine_read_psf (1234-1618): types (one per atom type and charge),
+     particles from the PDB (Angstrom, wrapped into the box), bonds, angles,
+     dihedrals and impropers into the topology lists;
+   - engine_read_cpf (630-1232, CHARMM parameter file) and engine_read_xplor
+     (82-628): the NONBONDED parameters (eps in kcal/mol, Rmin/2 in A) become
+     Lennard-Jones tables for every type pair, with Coulomb (kappa == 0),
+     real-space Ewald (kappa > 0) or none (kappa < 0) added for charged
+     pairs; cpf uses the switched forms over switchlen.  With rigidH, bonds
+     to hydrogen types become rigid constraints and leave the bond list.
+     Bonded parameters are read and not used. */
+/* New helpers for inspection: the table of a type pair (NULL if none); a
+   type's mass, charge, eps and rmin (out[4]) and names (at least 64 chars
+   each, may be NULL); the topology lists (which: 0 bonds as i j, 1 angles
+   as i j k, 2 dihedrals as i j k l pid; out may be NULL), returning the
+   count. */
+struct potential *engine_getpot(struct engine *e, int i, int j);
+int engine_gettypeinfo(struct engine *e, int k, double *out, char *name, char *name2);
+int engine_gettopology(struct engine *e, int which, int *out);
+int engine_read_psf(struct engine *e, int psf, int pdb);
+int engine_read_cpf(struct engine *e, int cpf, double kappa, double tol, int rigidH, double switchlen);
+int engine_read_xplor(struct engine *e, int xplor, double kappa, double tol, int rigidH);
 /* New helper: s->maxdx and the per-cell maxdx (cell_maxdx may be NULL). */
 int engine_getmaxdx(struct engine *e, double *maxdx, double *cell_maxdx);
 /* New helper: by particle id, the cell-relative position (x, 3 per id) and

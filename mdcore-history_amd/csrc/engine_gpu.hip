@@ -20,7 +20,7 @@
  *               are (owner atom, partition); per neighbour cell the positions
  *               and sorted words are prefetched one neighbour ahead and the
  *               cell is staged in LDS in window order as packed records
- *               (x, y, z, key | index << 16 | type << 26).  Each lane walks
+ *               (x, y, z, key | index << 16 | type << tshift).  Each lane walks
  *               its share of the sorted window (runner_dopair.c:455-478),
  *               tests the cutoff (re-done in FP64 inside a thin band around
  *               rc^2) and keeps its hits in a lane-private LDS list, then
@@ -177,8 +177,8 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 /* Packed staged record: key (16 bits) | index in cell (10 bits) | type (6 bits). */
 __device__ __forceinline__ unsigned int rec_key(unsigned int w) { return w & 0xffffu; }
-__device__ __forceinline__ int rec_idx(unsigned int w) { return (int)((w >> 16) & 1023u); }
-__device__ __forceinline__ int rec_type(unsigned int w) { return (int)(w >> 26); }
+__device__ __forceinline__ int rec_idx(unsigned int w, int imask) { return (int)((w >> 16) & (unsigned int)imask); }
+__device__ __forceinline__ int rec_type(unsigned int w, int tshift) { return (int)(w >> tshift); }
 
 /* ------------------------------------------------------------ k_pack */
 
@@ -551,7 +551,7 @@ __device__ __forceinline__ void nonbond_body(
                 /* Stage the neighbour, in the order its window is walked:
                    ascending keys when the owner is the task's i cell,
                    descending otherwise, storage order for the self task.
-                   Record = (x, y, z, key' | index << 16 | type << 26) with
+                   Record = (x, y, z, key' | index << 16 | type << tshift) with
                    key' = key, or 0xffff - key when the owner is the j cell. */
                 wave_lds_sync();
                 if (lane < nj) raw[lane] = pfx;
@@ -560,14 +560,14 @@ __device__ __forceinline__ void nonbond_body(
                     if (self)
                         stage[lane] = make_float4(pfx.x, pfx.y, pfx.z,
                                                   __uint_as_float(((unsigned int)lane << 16) |
-                                                                  ((unsigned int)__float_as_int(pfx.w) << 26)));
+                                                                  ((unsigned int)__float_as_int(pfx.w) << P.tshift)));
                     else {
                         const int idx = (int)(pfw >> 16) < nj ? (int)(pfw >> 16) : nj - 1;
                         const float4 p = idx < 64 ? raw[idx] : xf[offj + idx];
                         const unsigned int kk = orient ? 0xffffu - (pfw & 0xffffu) : (pfw & 0xffffu);
                         stage[lane] = make_float4(p.x, p.y, p.z,
                                                   __uint_as_float(kk | ((unsigned int)idx << 16) |
-                                                                  ((unsigned int)__float_as_int(p.w) << 26)));
+                                                                  ((unsigned int)__float_as_int(p.w) << P.tshift)));
                     }
                 }
                 for (int t = lane + 64; t < nj; t += 64) {
@@ -575,7 +575,7 @@ __device__ __forceinline__ void nonbond_body(
                         const float4 p = xf[offj + t];
                         stage[t] = make_float4(p.x, p.y, p.z,
                                                __uint_as_float(((unsigned int)t << 16) |
-                                                               ((unsigned int)__float_as_int(p.w) << 26)));
+                                                               ((unsigned int)__float_as_int(p.w) << P.tshift)));
                     } else {
                         const unsigned int w =
                             sortlist[13 * (size_t)offj + (size_t)sid * nj + (orient ? t : nj - 1 - t)];
@@ -584,7 +584,7 @@ __device__ __forceinline__ void nonbond_body(
                         const unsigned int kk = orient ? 0xffffu - (w & 0xffffu) : (w & 0xffffu);
                         stage[t] = make_float4(p.x, p.y, p.z,
                                                __uint_as_float(kk | ((unsigned int)idx << 16) |
-                                                               ((unsigned int)__float_as_int(p.w) << 26)));
+                                                               ((unsigned int)__float_as_int(p.w) << P.tshift)));
                     }
                 }
                 wave_lds_sync();
@@ -664,7 +664,7 @@ __device__ __forceinline__ void nonbond_body(
                                     /* Decide in FP64 exactly as the reference does, in
                                        the task's own orientation. */
                                     const double *xa = &x64[3 * (size_t)a];
-                                    const double *xb = &x64[3 * (size_t)(offj + rec_idx(wj))];
+                                    const double *xb = &x64[3 * (size_t)(offj + rec_idx(wj, P.imask))];
                                     const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cj];
                                     double r2d;
                                     if (VK != 2)
@@ -692,8 +692,8 @@ __device__ __forceinline__ void nonbond_body(
                                 const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
                                 const float4 pa = stage[ta], pb = stage[tb];
                                 const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
-                                const float4 ha = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wa)));
-                                const float4 hb = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wb)));
+                                const float4 ha = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wa, P.tshift)));
+                                const float4 hb = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wb, P.tshift)));
                                 const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
                                             dza = (xi.z - pa.z) - sf2;
                                 const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1,
@@ -709,8 +709,8 @@ __device__ __forceinline__ void nonbond_body(
                                 lfy -= fa * dya + fb * dyb;
                                 lfz -= fa * dza + fb * dzb;
                                 if (VK != 2) {
-                                    const bool ca = ua && (VK == 1 || myidx > rec_idx(wa));
-                                    const bool cb = ub && (VK == 1 || myidx > rec_idx(wb));
+                                    const bool ca = ua && (VK == 1 || myidx > rec_idx(wa, P.imask));
+                                    const bool cb = ub && (VK == 1 || myidx > rec_idx(wb, P.imask));
                                     if (ca) {
                                         le += ea;
                                         lv -= fa * r2a;
@@ -735,12 +735,12 @@ __device__ __forceinline__ void nonbond_body(
                                 const int ta = hitl[s * 64 + lane];
                                 const float4 pa = stage[ta];
                                 const unsigned int wa = __float_as_uint(pa.w);
-                                const float4 ha = ptab[ti * mt + rec_type(wa)];
+                                const float4 ha = ptab[ti * mt + rec_type(wa, P.tshift)];
                                 if (__float_as_int(ha.w) == -1) continue;
                                 const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
                                             dza = (xi.z - pa.z) - sf2;
                                 const float r2a = dxa * dxa + dya * dya + dza * dza;
-                                const float rlo = N.sc[ptype[ti * mt + rec_type(wa)]].x;
+                                const float rlo = N.sc[ptype[ti * mt + rec_type(wa, P.tshift)]].x;
                                 if (r2a >= rlo * rlo) {
                                     hitl[nd * 64 + lane] = (unsigned short)ta;
                                     nd++;
@@ -751,7 +751,7 @@ __device__ __forceinline__ void nonbond_body(
                                 lfx -= fa * dxa;
                                 lfy -= fa * dya;
                                 lfz -= fa * dza;
-                                if (VK != 2 && (VK == 1 || myidx > rec_idx(wa))) {
+                                if (VK != 2 && (VK == 1 || myidx > rec_idx(wa, P.imask))) {
                                     le += ea;
                                     lv -= fa * r2a;
                                     lnp++;
@@ -767,7 +767,7 @@ __device__ __forceinline__ void nonbond_body(
                                 const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
                                             dza = (xi.z - pa.z) - sf2;
                                 dn_r2[off + s] = dxa * dxa + dya * dya + dza * dza;
-                                dn_p[off + s] = ptype[ti * mt + rec_type(__float_as_uint(pa.w))];
+                                dn_p[off + s] = ptype[ti * mt + rec_type(__float_as_uint(pa.w), P.tshift)];
                             }
                             wave_lds_sync();
                             mlp_dense_eval(N, dn_r2, dn_p, dn_f, dn_e, T, lane);
@@ -782,7 +782,7 @@ __device__ __forceinline__ void nonbond_body(
                                 lfx -= fa * dxa;
                                 lfy -= fa * dya;
                                 lfz -= fa * dza;
-                                if (VK != 2 && (VK == 1 || myidx > rec_idx(wa))) {
+                                if (VK != 2 && (VK == 1 || myidx > rec_idx(wa, P.imask))) {
                                     le += ea;
                                     lv -= fa * r2a;
                                     lnp++;
@@ -1219,7 +1219,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
     if (e->gpu != NULL) engine_gpu_finalize(e);
     s = &e->s;
     mt = e->max_type;
-    if (mt > MD_MAX_TYPES) return gerror_(engine_err_nyi, "more than 64 particle types");
+    if (mt > MD_MAX_TYPES) return gerror_(engine_err_nyi, "more than 1024 particle types");
     g = (struct gpu_state *)calloc(1, sizeof(struct gpu_state));
     if (g == NULL) return gerror_(engine_err_malloc, "calloc failed");
     e->gpu = g;
@@ -1323,6 +1323,14 @@ extern "C" int engine_gpu_load(struct engine *e) {
         P->dscale = (double)32767 / (2 * P->bias);
         P->dmaxdist = (int)(2 + P->dscale * (s->cutoff + 0.0 + 0.0));
         P->cutoff = s->cutoff;
+        /* Packed record: key (16 bits) | slot (ibits) | type (16 - ibits). */
+        {
+            int tbits = 6;
+            while ((1 << tbits) < mt) tbits += 2;
+            P->tshift = 32 - tbits;
+            P->imask = (1 << (16 - tbits)) - 1;
+            g->maxcell = 1 << (16 - tbits);
+        }
         P->maxdx = NULL;
         P->cutoff2 = s->cutoff * s->cutoff;
         for (k = 0; k < 3; k++) {
@@ -1489,7 +1497,9 @@ int gpu_upload(struct engine *e) {
         if (c > maxc) maxc = c;
         n += c;
     }
-    if (maxc > MD_MAX_CELL) return gerror_(engine_err_maxparts, "more than 1024 particles in a cell");
+    if (maxc > g->maxcell)
+        return gerror_(engine_err_maxparts, "more particles in a cell than the packed record holds (1024 for up to "
+                                            "64 types, 256 for up to 256, 64 for up to 1024)");
     if (gpu_alloc_parts(g, n > 0 ? n : 1) < 0) return engine_err_gpu;
     for (cid = 0; cid < s->nr_cells; cid++) {
         struct cell *c = &s->cells[cid];

@@ -11,8 +11,8 @@
 #include "mlp_tile.h"
 
 #define MD_NQUEUE 8
-#define MD_MAX_CELL 1024 /* particle index in a packed record: 10 bits */
-#define MD_MAX_TYPES 64  /* particle type in a packed record: 6 bits */
+#define MD_MAX_CELL 1024  /* particles per cell: the packed record's slot field is 10 bits at most */
+#define MD_MAX_TYPES 1024 /* particle types: 6, 8 or 10 type bits, leaving 10, 8 or 6 slot bits */
 
 #define gerror_(id, msg) errs_register((id), (msg), __LINE__, __func__, __FILE__)
 
@@ -34,6 +34,7 @@ struct nb_params {
     double dim[3], h[3];
     float rc2f, band;
     double cutoff;
+    int tshift, imask; /* packed record: type = w >> tshift, slot = (w >> 16) & imask */
     /* Pseudo-Verlet: per-cell maxdx widening each pair window
        (runner_dopair.c:428), or NULL (no widening: dmaxdist). */
     const double *maxdx;
@@ -54,6 +55,7 @@ struct gpu_state {
     hipStream_t stream;
     hipEvent_t ev[8];
     int ncells, ntab, max_type, nparts_cap, nids_cap, maxcount, maxnbr;
+    int maxcell; /* particles per cell the packed record holds */
     int nparts, nids;
     struct nb_params P;
     int grid_nb[3], ncu; /* [0] table mode, [1] surrogate mode, [2] table mode, 5-wave budget */

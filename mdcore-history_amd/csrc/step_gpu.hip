@@ -710,7 +710,7 @@ static int step_status(struct gpu_state *g, struct step_bufs *b, int *maxc, int 
         return gerror_(engine_err_gpu, "migration failed");
     *maxc = st[0];
     *outside = st[1];
-    if (st[0] > MD_MAX_CELL) return gerror_(engine_err_maxparts, "more than 1024 particles in a cell");
+    if (st[0] > g->maxcell) return gerror_(engine_err_maxparts, "more particles in a cell than the packed record holds");
     return engine_err_ok;
 }
 

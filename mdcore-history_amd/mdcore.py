@@ -90,6 +90,14 @@ def library():
         "engine_rigid_add": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
         "engine_rigid_eval": (C.c_int, [C.c_void_p]),
         "engine_getrigid": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp]),
+        "engine_read_psf": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "engine_read_cpf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double]),
+        "engine_read_xplor": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
+        "engine_getpot": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int]),
+        "engine_gettypeinfo": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_char_p, C.c_char_p]),
+        "engine_gettopology": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+        "engine_gettype": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "engine_gettype2": (C.c_int, [C.c_void_p, C.c_char_p]),
         "engine_advance": (C.c_int, [C.c_void_p]),
         "engine_gettasks": (C.c_int, [C.c_void_p, _ip, C.c_int]),
         "engine_prepare": (C.c_int, [C.c_void_p]),
@@ -356,6 +364,53 @@ class Engine:
         cm = np.zeros(max(ncells, 1))
         rb = check(self.lib.engine_getmaxdx(self.e, ptr(m), ptr(cm)), "engine_getmaxdx")
         return float(m[0]), cm[:ncells], rb
+
+    # ------------------------------------------------------------ readers
+
+    def read_psf(self, psf, pdb):
+        """engine_read_psf on the two files; particles join the engine."""
+        fp, fd = os.open(str(psf), os.O_RDONLY), os.open(str(pdb), os.O_RDONLY)
+        try:
+            check(self.lib.engine_read_psf(self.e, fp, fd), "engine_read_psf")
+        finally:
+            os.close(fp)
+            os.close(fd)
+        self.N = check(self.lib.engine_getcells(self.e, None, None), "engine_getcells")
+
+    def read_cpf(self, path, kappa, tol, rigidH=0, switchlen=0.0):
+        f = os.open(str(path), os.O_RDONLY)
+        try:
+            check(self.lib.engine_read_cpf(self.e, f, kappa, tol, int(rigidH), switchlen), "engine_read_cpf")
+        finally:
+            os.close(f)
+
+    def read_xplor(self, path, kappa, tol, rigidH=0):
+        f = os.open(str(path), os.O_RDONLY)
+        try:
+            check(self.lib.engine_read_xplor(self.e, f, kappa, tol, int(rigidH)), "engine_read_xplor")
+        finally:
+            os.close(f)
+
+    def pot(self, i, j):
+        """The table of type pair (i, j), or None (borrowed: owned by the engine)."""
+        h = self.lib.engine_getpot(self.e, i, j)
+        return Potential(h, f"pair {i} {j}") if h else None
+
+    def typeinfo(self, k):
+        out = np.zeros(4)
+        n1 = C.create_string_buffer(64)
+        n2 = C.create_string_buffer(64)
+        nt = check(self.lib.engine_gettypeinfo(self.e, k, ptr(out), n1, n2), "engine_gettypeinfo")
+        return {"mass": out[0], "charge": out[1], "eps": out[2], "rmin": out[3], "name": n1.value.decode(),
+                "name2": n2.value.decode(), "ntypes": nt}
+
+    def topology(self, which):
+        """0: bonds (i, j); 1: angles (i, j, k); 2: dihedrals (i, j, k, l, pid)."""
+        width = (2, 3, 5)[which]
+        n = check(self.lib.engine_gettopology(self.e, which, None), "engine_gettopology")
+        out = np.zeros(max(n, 1) * width, dtype=np.int32)
+        check(self.lib.engine_gettopology(self.e, which, ptr(out, C.c_int)), "engine_gettopology")
+        return out[:n * width].reshape(-1, width)
 
     def rigid_add(self, i, j, d):
         check(self.lib.engine_rigid_add(self.e, int(i), int(j), float(d)), "engine_rigid_add")

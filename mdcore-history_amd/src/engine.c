@@ -451,6 +451,13 @@ int engine_finalize(struct engine *e) {
     free(e->exclusions);
     free(e->rigids);
     free(e->part2rigid);
+    free(e->bonds);
+    free(e->angles);
+    free(e->dihedrals);
+    e->bonds = NULL;
+    e->angles = NULL;
+    e->dihedrals = NULL;
+    e->nr_bonds = e->bonds_size = e->nr_angles = e->angles_size = e->nr_dihedrals = e->dihedrals_size = 0;
     e->rigids = NULL;
     e->part2rigid = NULL;
     e->nr_rigids = e->rigids_size = e->part2rigid_size = 0;

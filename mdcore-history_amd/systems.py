@@ -161,14 +161,18 @@ def tables(sys):
     pind = -np.ones(mt * mt, dtype=np.int32)
     ns, alphas, cs, offs = [], [], [], []
     off = 0
-    for k, (ti, tj, p) in enumerate(sys.pots):
-        d = p.data()
+    index = {}  # one table per distinct potential object
+    for ti, tj, p in sys.pots:
+        k = index.get(id(p))
+        if k is None:
+            d = p.data()
+            k = index[id(p)] = len(ns)
+            ns.append(d["n"])
+            alphas.append(d["alpha"])
+            cs.append(d["c"])
+            offs.append(off)
+            off += len(d["c"])
         pind[ti * mt + tj] = k
         pind[tj * mt + ti] = k
-        ns.append(d["n"])
-        alphas.append(d["alpha"])
-        cs.append(d["c"])
-        offs.append(off)
-        off += len(d["c"])
     return {"max_type": mt, "pind": pind, "n": np.array(ns, np.int32), "alpha": np.concatenate(alphas),
             "c": np.concatenate(cs), "off": np.array(offs, np.int32)}

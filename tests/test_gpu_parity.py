@@ -28,7 +28,26 @@ SYSTEMS = {
     # ~330 atoms/cell: six chunks per wave and the sort path for cells too
     # large for the LDS sort
     "dense_bigcells": lambda: dense_big_cells(),
+    # more types than 6 bits hold: the packed record moves to 8 type bits
+    # (<= 256 particles per cell), then to 10 (<= 64 per cell)
+    "many_types_100": lambda: many_types(100),
+    "many_types_600": lambda: many_types(600),
 }
+
+
+def many_types(ntypes, N=4000, nbase=5, seed=13):
+    """The 4k mixture's geometry with ntypes particle types; type t has the
+    parameters of base species t % nbase, so only 15 distinct tables."""
+    base = systems.mixture(N, seed=seed, nspecies=nbase)
+    types = md.gen_types(N, ntypes, 1, seed + 1)
+    pot_of = {}
+    for ti, tj, p in base.pots:
+        pot_of[(ti - 1, tj - 1)] = p
+        pot_of[(tj - 1, ti - 1)] = p
+    pots = [(i, j, pot_of[((i - 1) % nbase, (j - 1) % nbase)]) for i in range(1, ntypes + 1)
+            for j in range(i, ntypes + 1)]
+    masses = np.concatenate([[0.0], [base.masses[1 + (t % nbase)] for t in range(ntypes)]])
+    return systems.System("many_types", base.origin, base.dim, base.L, base.cutoff, base.x, types, masses, pots)
 
 
 def dense_big_cells(L=3.9, density=150.0, seed=3):
