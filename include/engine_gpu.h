@@ -102,6 +102,17 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
    r_lo[p] <= r <= cutoff through the network and the others through the
    tables. */
 #define engine_flag_mlp 131072
+/* Pair-kernel shell, chosen at engine_gpu_load (no reference counterpart):
+   full shell    every pair evaluated by both of its atoms' cells, forces
+                 kept in registers;
+   half shell    every pair task visited once, the opposite forces added in
+                 LDS per visit and folded per atom in a fixed order.
+   Without either flag the half shell is taken when the cells hold
+   engine_gpu_half_mean atoms or more on average (it saves work there; with
+   few atoms per cell its per-visit cost outweighs the halved pair work). */
+#define engine_flag_fullshell 262144
+#define engine_flag_halfshell 524288
+#define engine_gpu_half_mean 48
 
 /* Species pairs: ti[p], tj[p] (ti <= tj) for p < returned count. */
 int engine_mlp_pairs(struct engine *e, int *ti, int *tj, int max);

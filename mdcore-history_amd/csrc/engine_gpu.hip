@@ -381,6 +381,10 @@ struct nb_out {
     float4 *fhi, *flo;
     double *cell_acc;
     unsigned long long *cell_cnt;
+    /* half-shell mode: per visit (neighbour-list entry), the forces on the
+       visited cell's atoms, jcap slots per entry */
+    float4 *jpart;
+    int jcap;
 };
 
 /* True if p holds on any lane (scalar test of the wave's ballot mask). */
@@ -439,7 +443,7 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
                 adds its own contiguous run of results in order, so the
                 result stays deterministic.  Excluded pairs take the same
                 route, so a pair and its exclusion cancel exactly. */
-template <bool MLP>
+template <bool MLP, bool HALF>
 __device__ __forceinline__ void nonbond_body(
     unsigned char *smem, struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
     const int *__restrict__ nbr_off, const float4 *__restrict__ nbs, const double *__restrict__ origin,
@@ -462,11 +466,15 @@ __device__ __forceinline__ void nonbond_body(
     float *dn_f = (float *)(smem + P.lds_dense + 8 * 64 * MD_HCAP_MLP);
     float *dn_e = (float *)(smem + P.lds_dense + 12 * 64 * MD_HCAP_MLP);
     int *exb = (int *)(smem + P.lds_dense + 16 * 64 * MD_HCAP_MLP);      /* [XCAP*64]   */
+    /* half-shell mode: forces on the visited cell's records, 3 per record */
+    float *jf = (float *)(smem + P.lds_jf);                              /* [4][3*maxn] */
     const int lane = threadIdx.x;
     const int mt = P.max_type;
     const __amdgpu_buffer_rsrc_t rs_ptab = make_rsrc(ptab, P.ptab_bytes);
     const __amdgpu_buffer_rsrc_t rs_coef = make_rsrc(coeffs4, P.coef_bytes);
 
+    if (HALF) /* the visited cell's force copies start at zero; each visit's write-out clears them again */
+        for (int z = lane; z < 4 * 3 * P.maxn; z += 64) jf[z] = 0.f;
     int q = blockIdx.x % MD_NQUEUE, tried = 0;
     while (tried < MD_NQUEUE) {
         const int lo_q = (int)(((long long)q * P.ncells) / MD_NQUEUE);
@@ -654,6 +662,8 @@ __device__ __forceinline__ void nonbond_body(
                             const unsigned int wj = __float_as_uint(pj.w);
                             alive = alive && t < nj;
                             if (VK != 0) alive = alive && (int)rec_key(wj) < kmax;
+                            /* half shell: a self task takes each pair once, j < i */
+                            if (HALF && VK == 0) alive = alive && t < myidx;
                             const bool cand = VK == 0 ? (alive && t != myidx) : alive;
                             const float dx = (xi.x - pj.x) - sf0, dy = (xi.y - pj.y) - sf1, dz = (xi.z - pj.z) - sf2;
                             const float r2 = dx * dx + dy * dy + dz * dz;
@@ -685,11 +695,18 @@ __device__ __forceinline__ void nonbond_body(
                         }
 
                         if constexpr (!MLP) {
-                            /* Evaluate the kept hits, two per lane per round. */
+                            /* Evaluate the kept hits, two per lane per round.  In
+                               half-shell mode each lane starts its list at il mod h,
+                               so that lanes of one partition (which kept hits from
+                               the same records) add to different records at once. */
+                            const int rot = (HALF && h > 0) ? il % h : 0;
                             for (int s = 0; wave_any(s < h); s += 2) {
                                 const bool va = s < h, vb = s + 1 < h;
-                                const int ta = va ? hitl[s * 64 + lane] : 0;
-                                const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
+                                int sa = s + rot, sb = s + 1 + rot;
+                                if (sa >= h) sa -= h;
+                                if (sb >= h) sb -= h;
+                                const int ta = va ? hitl[sa * 64 + lane] : 0;
+                                const int tb = vb ? hitl[sb * 64 + lane] : ta;
                                 const float4 pa = stage[ta], pb = stage[tb];
                                 const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
                                 const float4 ha = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wa, P.tshift)));
@@ -708,6 +725,21 @@ __device__ __forceinline__ void nonbond_body(
                                 lfx -= fa * dxa + fb * dxb;
                                 lfy -= fa * dya + fb * dyb;
                                 lfz -= fa * dza + fb * dzb;
+                                if constexpr (HALF) {
+                                    /* the opposite force on the record's atom, into this
+                                       lane's copy (lane mod 4) of the visit's array */
+                                    float *jc = jf + (lane & 3) * 3 * P.maxn;
+                                    if (ua) {
+                                        atomicAdd(&jc[3 * ta], fa * dxa);
+                                        atomicAdd(&jc[3 * ta + 1], fa * dya);
+                                        atomicAdd(&jc[3 * ta + 2], fa * dza);
+                                    }
+                                    if (ub) {
+                                        atomicAdd(&jc[3 * tb], fb * dxb);
+                                        atomicAdd(&jc[3 * tb + 1], fb * dyb);
+                                        atomicAdd(&jc[3 * tb + 2], fb * dzb);
+                                    }
+                                }
                                 if (VK != 2) {
                                     const bool ca = ua && (VK == 1 || myidx > rec_idx(wa, P.imask));
                                     const bool cb = ub && (VK == 1 || myidx > rec_idx(wb, P.imask));
@@ -800,6 +832,29 @@ __device__ __forceinline__ void nonbond_body(
                     visit(visit_kind<1>{});
                 else
                     visit(visit_kind<2>{});
+                if constexpr (HALF) {
+                    /* this visit's forces on the visited atoms, by slot; later
+                       chunks of the owner add to the first one's */
+                    wave_lds_sync();
+                    for (int q = lane; q < nj; q += 64) {
+                        const int idx = rec_idx(__float_as_uint(stage[q].w), P.imask);
+                        /* sum the 4 copies, and leave them zero for the next visit */
+                        float vv[3];
+                        for (int m = 0; m < 3; m++) {
+                            float *j0 = jf + 3 * q + m;
+                            const int s = 3 * P.maxn;
+                            vv[m] = (j0[0] + j0[s]) + (j0[2 * s] + j0[3 * s]);
+                            j0[0] = j0[s] = j0[2 * s] = j0[3 * s] = 0.f;
+                        }
+                        float4 v = make_float4(vv[0], vv[1], vv[2], 0.f);
+                        float4 *dst = &O.jpart[(size_t)(k0 + k) * O.jcap + idx];
+                        if (ch > 0) {
+                            const float4 o = *dst;
+                            v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, 0.f);
+                        }
+                        *dst = v;
+                    }
+                }
                 acc2(hx, lx, lfx);
                 acc2(hy, ly, lfy);
                 acc2(hz, lz, lfz);
@@ -998,7 +1053,7 @@ __device__ __forceinline__ void nonbond_body(
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbond(NB_PARAMS) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const struct mlp_net N = {};
-    nonbond_body<false>(smem, NB_ARGS, N, nullptr);
+    nonbond_body<false, false>(smem, NB_ARGS, N, nullptr);
 }
 
 /* Table mode for systems with many more cells than resident waves: the same
@@ -1008,13 +1063,58 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_wide(NB_PARAMS) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const struct mlp_net N = {};
-    nonbond_body<false>(smem, NB_ARGS, N, nullptr);
+    nonbond_body<false, false>(smem, NB_ARGS, N, nullptr);
 }
 
 /* Surrogate mode (engine_flag_mlp). */
 __global__ __launch_bounds__(64) void k_nonbond_mlp(NB_PARAMS, struct mlp_net N, const int *__restrict__ ptype) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    nonbond_body<true>(smem, NB_ARGS, N, ptype);
+    nonbond_body<true, false>(smem, NB_ARGS, N, ptype);
+}
+
+/* Half shell: each pair task once, from its i cell, with the opposite
+   forces on the j cell's atoms left per visit for k_fold. */
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbond_half(NB_PARAMS) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const struct mlp_net N = {};
+    nonbond_body<false, true>(smem, NB_ARGS, N, nullptr);
+}
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_half_wide(NB_PARAMS) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const struct mlp_net N = {};
+    nonbond_body<false, true>(smem, NB_ARGS, N, nullptr);
+}
+
+/* Half shell, second step: each atom's force = its owner's (hi, lo) plus
+   the forces its cell received as the j cell of other cells' tasks and in
+   its own self task, added with two-sum in a fixed order (the self entry,
+   then the incoming entries in ascending order), one wave per cell. */
+__global__ __launch_bounds__(64) void k_fold(int ncells, const int2 *__restrict__ cnt_off,
+                                             const int *__restrict__ self_entry, const int *__restrict__ inv_off,
+                                             const int *__restrict__ inv_list, const float4 *__restrict__ jpart,
+                                             int jcap, float4 *__restrict__ fhi, float4 *__restrict__ flo) {
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int2 co = cnt_off[c];
+    const int q0 = inv_off[c], q1 = inv_off[c + 1], es = self_entry[c];
+    for (int s = lane; s < co.x; s += 64) {
+        const int a = co.y + s;
+        const float4 h = fhi[a], l = flo[a];
+        float hx = h.x, hy = h.y, hz = h.z, lx = l.x, ly = l.y, lz = l.z;
+        if (es >= 0) {
+            const float4 v = jpart[(size_t)es * jcap + s];
+            acc2(hx, lx, v.x);
+            acc2(hy, ly, v.y);
+            acc2(hz, lz, v.z);
+        }
+        for (int q = q0; q < q1; q++) {
+            const float4 v = jpart[(size_t)inv_list[q] * jcap + s];
+            acc2(hx, lx, v.x);
+            acc2(hy, ly, v.y);
+            acc2(hz, lz, v.z);
+        }
+        fhi[a] = make_float4(hx, hy, hz, 0.0f);
+        flo[a] = make_float4(lx, ly, lz, 0.0f);
+    }
 }
 
 /* ------------------------------------------------------------ k_reduce */
@@ -1150,6 +1250,13 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_nbr);
     (void)hipFree(g->d_nbs);
     (void)hipFree(g->d_nbr_off);
+    (void)hipFree(g->d_nbr_h);
+    (void)hipFree(g->d_nbs_h);
+    (void)hipFree(g->d_nbr_off_h);
+    (void)hipFree(g->d_self_entry);
+    (void)hipFree(g->d_inv_off);
+    (void)hipFree(g->d_inv_list);
+    (void)hipFree(g->d_jpart);
     (void)hipFree(g->d_origin);
     (void)hipFree(g->d_loc);
     (void)hipFree(g->d_sortflags);
@@ -1400,6 +1507,94 @@ extern "C" int engine_gpu_load(struct engine *e) {
         free(nbs);
     }
 
+    /* Half-shell lists: the self task and each pair task once, at the task's
+       i cell, in task order; each cell's self entry; and for each cell the
+       pair entries that visit it as the task's j cell, in task order. */
+    {
+        const int nc = s->nr_cells;
+        int *hcnt = (int *)calloc(nc + 1, sizeof(int)), *icnt = (int *)calloc(nc + 1, sizeof(int));
+        int *hoff = (int *)malloc(sizeof(int) * (nc + 1)), *ioff = (int *)malloc(sizeof(int) * (nc + 1));
+        int *hfill = (int *)malloc(sizeof(int) * (nc + 1)), *ifill = (int *)malloc(sizeof(int) * (nc + 1));
+        int *self_e = (int *)malloc(sizeof(int) * (nc > 0 ? nc : 1));
+        int htot = 0, itot = 0, bad = 0;
+        int4 *nbr = NULL;
+        float4 *nbs = NULL;
+        int *ilist = NULL;
+        if (hcnt == NULL || icnt == NULL || hoff == NULL || ioff == NULL || hfill == NULL || ifill == NULL ||
+            self_e == NULL)
+            bad = 1;
+        for (k = 0; !bad && k < s->nr_tasks; k++) {
+            const struct task *t = &s->tasks[k];
+            if (t->type == task_type_self)
+                hcnt[t->i]++;
+            else if (t->type == task_type_pair) {
+                hcnt[t->i]++;
+                icnt[t->j]++;
+            }
+        }
+        if (!bad) {
+            hoff[0] = ioff[0] = 0;
+            for (cid = 0; cid < nc; cid++) {
+                hoff[cid + 1] = hoff[cid] + hcnt[cid];
+                ioff[cid + 1] = ioff[cid] + icnt[cid];
+                self_e[cid] = -1;
+            }
+            htot = hoff[nc];
+            itot = ioff[nc];
+            memcpy(hfill, hoff, sizeof(int) * (nc + 1));
+            memcpy(ifill, ioff, sizeof(int) * (nc + 1));
+            nbr = (int4 *)malloc(sizeof(int4) * (htot > 0 ? htot : 1));
+            nbs = (float4 *)malloc(sizeof(float4) * (htot > 0 ? htot : 1));
+            ilist = (int *)malloc(sizeof(int) * (itot > 0 ? itot : 1));
+            if (nbr == NULL || nbs == NULL || ilist == NULL) bad = 1;
+        }
+        for (k = 0; !bad && k < s->nr_tasks; k++) {
+            const struct task *t = &s->tasks[k];
+            if (t->type == task_type_self) {
+                const int en = hfill[t->i]++;
+                self_e[t->i] = en;
+                nbs[en] = make_float4(0.f, 0.f, 0.f, 0.f);
+                nbr[en] = make_int4(t->i, 1 << 5, 0, 0);
+            } else if (t->type == task_type_pair) {
+                int dr, dp;
+                const int en = hfill[t->i]++;
+                task_dnshift(s, t->i, t->j, t->flags, g->P.dscale, &dr, &dp);
+                nbs[en] = cell_shift32(s, t->i, t->j);
+                nbr[en] = make_int4(t->j, t->flags, dr, dp);
+                ilist[ifill[t->j]++] = en;
+            }
+        }
+        if (!bad) {
+            g->n_entries_h = htot;
+            if (hipMalloc(&g->d_nbr_h, sizeof(int4) * (htot > 0 ? htot : 1)) != hipSuccess ||
+                hipMalloc(&g->d_nbs_h, sizeof(float4) * (htot > 0 ? htot : 1)) != hipSuccess ||
+                hipMalloc(&g->d_nbr_off_h, sizeof(int) * (nc + 1)) != hipSuccess ||
+                hipMalloc(&g->d_self_entry, sizeof(int) * (nc > 0 ? nc : 1)) != hipSuccess ||
+                hipMalloc(&g->d_inv_off, sizeof(int) * (nc + 1)) != hipSuccess ||
+                hipMalloc(&g->d_inv_list, sizeof(int) * (itot > 0 ? itot : 1)) != hipSuccess ||
+                hipMemcpy(g->d_nbr_h, nbr, sizeof(int4) * (htot > 0 ? htot : 1), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(g->d_nbs_h, nbs, sizeof(float4) * (htot > 0 ? htot : 1), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(g->d_nbr_off_h, hoff, sizeof(int) * (nc + 1), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(g->d_self_entry, self_e, sizeof(int) * (nc > 0 ? nc : 1), hipMemcpyHostToDevice) !=
+                    hipSuccess ||
+                hipMemcpy(g->d_inv_off, ioff, sizeof(int) * (nc + 1), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(g->d_inv_list, ilist, sizeof(int) * (itot > 0 ? itot : 1), hipMemcpyHostToDevice) !=
+                    hipSuccess)
+                bad = 2;
+        }
+        free(hcnt);
+        free(icnt);
+        free(hoff);
+        free(ioff);
+        free(hfill);
+        free(ifill);
+        free(self_e);
+        free(nbr);
+        free(nbs);
+        free(ilist);
+        if (bad) return gerror_(bad == 1 ? engine_err_malloc : engine_err_gpu, "half-shell lists");
+    }
+
     /* Cell origins, locations and sort masks. */
     {
         double *org = (double *)malloc(sizeof(double) * 3 * s->nr_cells);
@@ -1523,6 +1718,9 @@ int gpu_upload(struct engine *e) {
     if (maxid + 1 > g->nids) return gerror_(engine_err_range, "particle id beyond the ids known at engine_gpu_load");
     g->nparts = n;
     g->maxcount = maxc;
+    g->half = (e->flags & engine_flag_halfshell) ? 1
+              : (e->flags & engine_flag_fullshell) ? 0
+                                                   : (long long)n >= (long long)engine_gpu_half_mean * s->nr_cells;
     g->P.nparts = n;
     g->P.nids = g->nids;
     g->P.maxn = maxc > 0 ? maxc : 1;
@@ -1541,8 +1739,10 @@ int gpu_upload(struct engine *e) {
 /* One device pass on the resident particles; mlp selects the surrogate
    mode of the pair kernel. */
 int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
-    /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget */
-    const int kern = mlp ? 1 : (g->ncells >= 4 * 16 * g->ncu ? 2 : 0);
+    /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget, 3 / 4
+       the half-shell table kernels */
+    const int wide = g->ncells >= 4 * 16 * g->ncu;
+    const int kern = mlp ? 1 : g->half ? (wide ? 4 : 3) : (wide ? 2 : 0);
     struct nb_params P = g->P;
     int nb;
     size_t lds = 0;
@@ -1561,7 +1761,19 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     lds = (lds + 15) & ~(size_t)15;
     P.lds_dense = (int)lds;
     if (mlp) lds += 16 * 64 * MD_HCAP_MLP + 4 * 64 * MD_XCAP_MLP;
+    lds = (lds + 15) & ~(size_t)15;
+    P.lds_jf = (int)lds;
+    if (kern >= 3) lds += sizeof(float) * 4 * 3 * maxn;
     if (lds > 64 * 1024) return gerror_(engine_err_maxparts, "LDS staging exceeds 64 KiB");
+    if (kern >= 3 && (g->d_jpart == NULL || g->jcap < (int)maxn)) {
+        /* per half-shell entry, room for the visited cell's atoms; no
+           pending pass may still use the old buffer */
+        HIPCHECK(hipStreamSynchronize(g->stream));
+        (void)hipFree(g->d_jpart);
+        g->d_jpart = NULL;
+        g->jcap = (int)maxn + 8;
+        HIPCHECK(hipMalloc(&g->d_jpart, sizeof(float4) * (size_t)g->jcap * (g->n_entries_h > 0 ? g->n_entries_h : 1)));
+    }
     if (lds != g->lds_nb[kern]) {
         /* Persistent grid: as many single-wave blocks as can be resident. */
         int per_cu = 0;
@@ -1569,6 +1781,10 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_mlp, 64, lds));
         else if (kern == 2)
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_wide, 64, lds));
+        else if (kern == 3)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_half, 64, lds));
+        else if (kern == 4)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_half_wide, 64, lds));
         else
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond, 64, lds));
         if (per_cu < 1) per_cu = 1;
@@ -1607,7 +1823,17 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     O.flo = g->d_flo;
     O.cell_acc = g->d_cell_acc;
     O.cell_cnt = g->d_cell_cnt;
-    if (kern == 1)
+    O.jpart = g->d_jpart;
+    O.jcap = g->jcap;
+    if (kern >= 3) {
+        hipLaunchKernelGGL(kern == 4 ? k_nonbond_half_wide : k_nonbond_half, dim3(g->grid_nb[kern]), dim3(64), lds,
+                           g->stream, P, g->d_cnt_off, g->d_nbr_h, g->d_nbr_off_h, g->d_nbs_h, g->d_origin, g->d_loc,
+                           g->d_xf, g->d_x, g->d_id, g->d_sortlist, g->d_keyof, g->d_ptab,
+                           (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list, g->d_index_of_id,
+                           g->d_cell_of, O, g->d_queue);
+        hipLaunchKernelGGL(k_fold, dim3(g->ncells), dim3(64), 0, g->stream, g->ncells, g->d_cnt_off, g->d_self_entry,
+                           g->d_inv_off, g->d_inv_list, (const float4 *)g->d_jpart, g->jcap, g->d_fhi, g->d_flo);
+    } else if (kern == 1)
         hipLaunchKernelGGL(k_nonbond_mlp, dim3(g->grid_nb[1]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,

@@ -41,7 +41,7 @@ struct nb_params {
     int maxn, maxnbr;
     int ptab_bytes, coef_bytes; /* sizes of the table headers and coefficients */
     /* dynamic LDS layout (byte offsets) */
-    int lds_own_key, lds_kfl, lds_stage, lds_hit, lds_dense;
+    int lds_own_key, lds_kfl, lds_stage, lds_hit, lds_dense, lds_jf;
 };
 
 /* Neighbour-list entry of an owner cell: x = other cell, y = sid | orient<<4
@@ -58,14 +58,24 @@ struct gpu_state {
     int maxcell; /* particles per cell the packed record holds */
     int nparts, nids;
     struct nb_params P;
-    int grid_nb[3], ncu; /* [0] table mode, [1] surrogate mode, [2] table mode, 5-wave budget */
-    size_t lds_nb[3];
+    /* [0] table mode, [1] surrogate mode, [2] table mode with the 5-wave
+       budget, [3] / [4] the half-shell table kernels with 4 / 5 waves */
+    int grid_nb[5], ncu;
+    size_t lds_nb[5];
     /* static */
     float4 *d_ptab;
     float *d_coeffs;
     int4 *d_nbr;
     float4 *d_nbs; /* per neighbour entry: FP32 minimum-image shift owner -> neighbour */
     int *d_nbr_off;
+    /* half shell: each task once at its i cell; the self entry of each cell;
+       the entries that visit each cell as a task's j cell; per entry the
+       forces on the visited atoms (jcap slots) */
+    int4 *d_nbr_h;
+    float4 *d_nbs_h;
+    int *d_nbr_off_h, *d_self_entry, *d_inv_off, *d_inv_list;
+    int n_entries_h, jcap, half;
+    float4 *d_jpart;
     double *d_origin;
     int *d_loc;
     unsigned int *d_sortflags;

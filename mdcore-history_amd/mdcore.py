@@ -30,6 +30,9 @@ engine_flag_verlet = 16
 engine_flag_shake = 1024
 engine_flag_verlet_pseudo = 256
 engine_flag_mlp = 131072
+engine_flag_fullshell = 262144
+engine_flag_halfshell = 524288
+engine_gpu_half_mean = 48
 
 space_periodic_full = 7
 

@@ -3,7 +3,9 @@
 Bit-exact: cell contents, every sorted list, pair counts (both windows),
 exclusion counts.  Floating point: per-atom force relative RMSE <= 1e-5 and
 energy / virial relative error <= 1e-6 against the FP64 oracle (FP32 pair
-arithmetic, two-sum accumulation, FP64 reductions).
+arithmetic, two-sum accumulation, FP64 reductions).  Every system runs with
+both pair-kernel shells (full: each pair at both cells; half: each pair task
+once, opposite forces folded per atom).
 """
 import numpy as np
 import pytest
@@ -58,12 +60,17 @@ def dense_big_cells(L=3.9, density=150.0, seed=3):
                           np.array([0.0, 1.0]), [(1, 1, pot)])
 
 
-@pytest.fixture(scope="module", params=sorted(SYSTEMS))
+SHELLS = {"full": md.engine_flag_fullshell, "half": md.engine_flag_halfshell}
+
+
+@pytest.fixture(scope="module", params=[(s, sh) for s in sorted(SYSTEMS) for sh in sorted(SHELLS)],
+                ids=lambda p: f"{p[0]}-{p[1]}")
 def case(request):
-    sys = SYSTEMS[request.param]()
+    name, shell = request.param
+    sys = SYSTEMS[name]()
     orc = po.from_system(sys)
-    eng = engine_pass(sys)
-    yield request.param, sys, orc, eng
+    eng = engine_pass(sys, md.engine_flag_gpu | SHELLS[shell])
+    yield f"{name}-{shell}", sys, orc, eng
     eng.close()
     orc.close()
 
@@ -116,7 +123,8 @@ def test_forces_energy_vs_fp64(case):
 
 def test_projected_window_finds_every_pair(case):
     name, sys, orc, _ = case
-    eng = engine_pass(sys, md.engine_flag_gpu | md.engine_flag_projwindow)
+    shell = SHELLS[name.rsplit("-", 1)[1]]
+    eng = engine_pass(sys, md.engine_flag_gpu | md.engine_flag_projwindow | shell)
     try:
         g = eng.gpu_results()
         p = orc.nonbond(po.OR_WINDOW_PROJ)
@@ -142,3 +150,21 @@ def test_repeatable(case):
     e2 = eng.gpu_results()
     assert np.array_equal(f1, f2), f"{name}: forces differ between passes"
     assert e1 == e2
+
+
+@pytest.mark.parametrize("name,expect", [("mixture_4k", "full"), ("water_1k", "half"), ("dense_bigcells", "half")])
+def test_shell_choice(name, expect):
+    """Without a shell flag the engine takes the half shell for cells of
+    engine_gpu_half_mean atoms or more on average: its forces are bitwise
+    those of the forced shell (the two shells add in different orders)."""
+    sys = SYSTEMS[name]()
+    mean = len(sys.x) / np.prod(np.floor(sys.dim / sys.L))
+    assert (mean >= md.engine_gpu_half_mean) == (expect == "half")
+    out = {}
+    for label, flags in (("auto", 0), ("full", md.engine_flag_fullshell), ("half", md.engine_flag_halfshell)):
+        eng = engine_pass(sys, md.engine_flag_gpu | flags)
+        out[label] = eng.forces().copy()
+        eng.close()
+    other = "half" if expect == "full" else "full"
+    assert np.array_equal(out["auto"], out[expect])
+    assert not np.array_equal(out["auto"], out[other])
