@@ -156,6 +156,49 @@ __device__ __forceinline__ void pot_eval32r(const float4 hdr, __amdgpu_buffer_rs
     f = eff * a.y * ir;
 }
 
+/* pot_eval32r for two pairs at once, without branches: both coefficient
+   rows are requested before either polynomial is evaluated, so the two
+   table lookups overlap.  A pair with use_* false (no hit, or no table for
+   its types) reads interval 0 of the first table instead and returns
+   e = f = 0; the arithmetic of a used pair is that of pot_eval32r. */
+__device__ __forceinline__ void pot_eval32r2(const float4 ha, const float4 hb, bool use_a, bool use_b,
+                                             __amdgpu_buffer_rsrc_t rc, float r2a, float r2b, float &ea,
+                                             float &fa, float &eb, float &fb) {
+    const float ira = __builtin_amdgcn_rsqf(r2a), irb = __builtin_amdgcn_rsqf(r2b);
+    const float ra = r2a * ira, rb = r2b * irb;
+    const unsigned int wa = use_a ? __float_as_uint(ha.w) : 0u;
+    const unsigned int wb = use_b ? __float_as_uint(hb.w) : 0u;
+    int ia = (int)fmaxf(0.0f, ha.x + ra * (ha.y + ra * ha.z));
+    int ib = (int)fmaxf(0.0f, hb.x + rb * (hb.y + rb * hb.z));
+    ia = ia > (int)(wa & 511u) ? (int)(wa & 511u) : ia;
+    ib = ib > (int)(wb & 511u) ? (int)(wb & 511u) : ib;
+    const int offa = 32 * ((int)(wa >> 9) + ia), offb = 32 * ((int)(wb >> 9) + ib);
+    const float4 a0 = ld_f4(rc, offa), a1 = ld_f4(rc, offa + 16);
+    const float4 b0 = ld_f4(rc, offb), b1 = ld_f4(rc, offb + 16);
+    auto horner = [](float r, float ir, const float4 c0, const float4 c1, float &e, float &f) {
+        const float x = (r - c0.x) * c0.y;
+        float ee = c0.z * x + c0.w;
+        float eff = c0.z;
+        eff = eff * x + ee;
+        ee = ee * x + c1.x;
+        eff = eff * x + ee;
+        ee = ee * x + c1.y;
+        eff = eff * x + ee;
+        ee = ee * x + c1.z;
+        eff = eff * x + ee;
+        ee = ee * x + c1.w;
+        e = ee;
+        f = eff * c0.y * ir;
+    };
+    float e1, f1, e2, f2;
+    horner(ra, ira, a0, a1, e1, f1);
+    horner(rb, irb, b0, b1, e2, f2);
+    ea = use_a ? e1 : 0.f;
+    fa = use_a ? f1 : 0.f;
+    eb = use_b ? e2 : 0.f;
+    fb = use_b ? f2 : 0.f;
+}
+
 /* The pair kernel's workgroup is a single wavefront, and the LDS executes a
    wavefront's instructions in order: data written by one lane is visible to
    another lane's later read once this wave's LDS operations have completed.
@@ -719,9 +762,8 @@ __device__ __forceinline__ void nonbond_body(
                                 const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
                                 const bool ua = va && __float_as_int(ha.w) != -1;
                                 const bool ub = vb && __float_as_int(hb.w) != -1;
-                                float ea = 0.f, fa = 0.f, eb = 0.f, fb = 0.f;
-                                if (ua) pot_eval32r(ha, rs_coef, r2a, ea, fa);
-                                if (ub) pot_eval32r(hb, rs_coef, r2b, eb, fb);
+                                float ea, fa, eb, fb;
+                                pot_eval32r2(ha, hb, ua, ub, rs_coef, r2a, r2b, ea, fa, eb, fb);
                                 lfx -= fa * dxa + fb * dxb;
                                 lfy -= fa * dya + fb * dyb;
                                 lfz -= fa * dza + fb * dzb;
