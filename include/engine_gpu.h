@@ -113,6 +113,13 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
 #define engine_flag_fullshell 262144
 #define engine_flag_halfshell 524288
 #define engine_gpu_half_mean 48
+/* Owner split (no reference counterpart): with fewer than 32 cells per
+   compute unit and engine_gpu_half_mean or more atoms per cell on average,
+   engine_gpu_load splits each owner cell's neighbour list into up to 8
+   groups, each a work item of the pair kernel with its own partial forces
+   and sums, folded per atom in group order.  engine_flag_nosplit keeps one
+   item per cell. */
+#define engine_flag_nosplit 1048576
 
 /* Species pairs: ti[p], tj[p] (ti <= tj) for p < returned count. */
 int engine_mlp_pairs(struct engine *e, int *ti, int *tj, int max);

@@ -428,6 +428,8 @@ struct nb_out {
        visited cell's atoms, jcap slots per entry */
     float4 *jpart;
     int jcap;
+    /* split owners (nsplit > 1): group g's forces on atom a at [g * nparts + a] */
+    float4 *phi, *plo;
 };
 
 /* True if p holds on any lane (scalar test of the wave's ballot mask). */
@@ -474,6 +476,8 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
 }
 
 #define MD_HCAP 32     /* table mode: hits buffered per lane before they are evaluated */
+#define MD_SPLIT_ITEMS_PER_CU 32 /* pair-kernel work items per CU below which owners are split */
+#define MD_SPLIT_MAX 8           /* at most this many neighbour groups per owner cell */
 #define MD_HCAP_MLP 8  /* MLP mode: hits per lane per compaction round */
 #define MD_XCAP_MLP 8  /* MLP mode: excluded partners per lane per round */
 
@@ -518,10 +522,12 @@ __device__ __forceinline__ void nonbond_body(
 
     if (HALF) /* the visited cell's force copies start at zero; each visit's write-out clears them again */
         for (int z = lane; z < 4 * 3 * P.maxn; z += 64) jf[z] = 0.f;
+    /* Work items: (owner cell, neighbour group), nsplit groups per cell. */
+    const int nitems = P.ncells * P.nsplit;
     int q = blockIdx.x % MD_NQUEUE, tried = 0;
     while (tried < MD_NQUEUE) {
-        const int lo_q = (int)(((long long)q * P.ncells) / MD_NQUEUE);
-        const int hi_q = (int)(((long long)(q + 1) * P.ncells) / MD_NQUEUE);
+        const int lo_q = (int)(((long long)q * nitems) / MD_NQUEUE);
+        const int hi_q = (int)(((long long)(q + 1) * nitems) / MD_NQUEUE);
         int it = 0;
         if (lane == 0) it = atomicAdd(&queue[q], 1);
         it = __shfl(it, 0);
@@ -531,19 +537,22 @@ __device__ __forceinline__ void nonbond_body(
             continue;
         }
         /* Wave-uniform from here: the neighbour list is read with scalar loads. */
-        const int c = __builtin_amdgcn_readfirstlane(lo_q + it);
+        const int item = __builtin_amdgcn_readfirstlane(lo_q + it);
+        const int c = item / P.nsplit, grp = item - c * P.nsplit;
         const int2 co_c = cnt_off[c];
         const int nc = co_c.x, offc = co_c.y;
         if (nc == 0) {
-            /* An empty cell still owns its slots of the per-cell sums. */
+            /* An empty cell still owns its slots of the per-item sums. */
             if (lane < 3) {
-                O.cell_acc[3 * c + lane] = 0.0;
-                O.cell_cnt[3 * c + lane] = 0ull;
+                O.cell_acc[3 * item + lane] = 0.0;
+                O.cell_cnt[3 * item + lane] = 0ull;
             }
             continue;
         }
         const double oc0 = origin[3 * c], oc1 = origin[3 * c + 1], oc2 = origin[3 * c + 2];
-        const int k0 = nbr_off[c], nn = nbr_off[c + 1] - k0;
+        /* this item's share of the owner's neighbour entries */
+        const int kc = nbr_off[c], nall = nbr_off[c + 1] - kc;
+        const int k0 = kc + (grp * nall) / P.nsplit, nn = kc + ((grp + 1) * nall) / P.nsplit - k0;
 
         /* Owner metadata: first/last key per direction. */
         wave_lds_sync();
@@ -577,7 +586,7 @@ __device__ __forceinline__ void nonbond_body(
             /* Prefetch the first neighbour: unsorted positions + sorted words. */
             float4 pfx = make_float4(0.f, 0.f, 0.f, 0.f);
             unsigned int pfw = 0;
-            {
+            if (nn > 0) {
                 const int4 en = nbr[k0];
                 const int2 cj = cnt_off[en.x];
                 if (lane < cj.x) {
@@ -909,7 +918,7 @@ __device__ __forceinline__ void nonbond_body(
             /* Excluded pairs: subtract what the pair pass added
                (exclusion.c:139-208), same FP32 arithmetic as the pair pass. */
             if constexpr (!MLP) {
-                if (active && part == 0) {
+                if (active && part == 0 && grp == 0) {
                     const int myid = pid[a];
                     for (int k = excl_off[myid]; k < excl_off[myid + 1]; k++) {
                         const int oid = excl_list[k];
@@ -946,7 +955,7 @@ __device__ __forceinline__ void nonbond_body(
                     }
                 }
             } else {
-                const bool xl = active && part == 0;
+                const bool xl = active && part == 0 && grp == 0;
                 const int myid = xl ? pid[a] : 0;
                 int kx = xl ? excl_off[myid] : 0;
                 const int kend = xl ? excl_off[myid + 1] : 0;
@@ -1040,15 +1049,20 @@ __device__ __forceinline__ void nonbond_body(
                 }
             }
             if (part == 0) {
-                O.fhi[a] = make_float4(hx, hy, hz, 0.0f);
-                O.flo[a] = make_float4(lx, ly, lz, 0.0f);
+                if (P.nsplit == 1) {
+                    O.fhi[a] = make_float4(hx, hy, hz, 0.0f);
+                    O.flo[a] = make_float4(lx, ly, lz, 0.0f);
+                } else {
+                    O.phi[(size_t)grp * P.nparts + a] = make_float4(hx, hy, hz, 0.0f);
+                    O.plo[(size_t)grp * P.nparts + a] = make_float4(lx, ly, lz, 0.0f);
+                }
             }
             if (!active) {
                 epot = vir = eexcl = 0.0;
                 np = nref = nexcl = 0;
             }
 
-            /* Per-cell sums; this wave owns the cell's slots, chunks add in order. */
+            /* Per-item sums; this wave owns the item's slots, chunks add in order. */
             epot = wave_sum(epot);
             vir = wave_sum(vir);
             eexcl = wave_sum(eexcl);
@@ -1057,19 +1071,19 @@ __device__ __forceinline__ void nonbond_body(
             nexcl = (unsigned int)wave_sum((int)nexcl);
             if (lane == 0) {
                 if (ch == 0) {
-                    O.cell_acc[3 * c] = epot;
-                    O.cell_acc[3 * c + 1] = eexcl;
-                    O.cell_acc[3 * c + 2] = vir;
-                    O.cell_cnt[3 * c] = np;
-                    O.cell_cnt[3 * c + 1] = nexcl;
-                    O.cell_cnt[3 * c + 2] = nref;
+                    O.cell_acc[3 * item] = epot;
+                    O.cell_acc[3 * item + 1] = eexcl;
+                    O.cell_acc[3 * item + 2] = vir;
+                    O.cell_cnt[3 * item] = np;
+                    O.cell_cnt[3 * item + 1] = nexcl;
+                    O.cell_cnt[3 * item + 2] = nref;
                 } else {
-                    O.cell_acc[3 * c] += epot;
-                    O.cell_acc[3 * c + 1] += eexcl;
-                    O.cell_acc[3 * c + 2] += vir;
-                    O.cell_cnt[3 * c] += np;
-                    O.cell_cnt[3 * c + 1] += nexcl;
-                    O.cell_cnt[3 * c + 2] += nref;
+                    O.cell_acc[3 * item] += epot;
+                    O.cell_acc[3 * item + 1] += eexcl;
+                    O.cell_acc[3 * item + 2] += vir;
+                    O.cell_cnt[3 * item] += np;
+                    O.cell_cnt[3 * item + 1] += nexcl;
+                    O.cell_cnt[3 * item + 2] += nref;
                 }
             }
         }
@@ -1127,21 +1141,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     nonbond_body<false, true>(smem, NB_ARGS, N, nullptr);
 }
 
-/* Half shell, second step: each atom's force = its owner's (hi, lo) plus
-   the forces its cell received as the j cell of other cells' tasks and in
-   its own self task, added with two-sum in a fixed order (the self entry,
-   then the incoming entries in ascending order), one wave per cell. */
-__global__ __launch_bounds__(64) void k_fold(int ncells, const int2 *__restrict__ cnt_off,
+/* Second step of a split and/or half-shell pass, one wave per cell: each
+   atom's force = its owner's (hi, lo), or with nsplit > 1 the groups'
+   (hi, lo) in group order, plus (half shell) the forces its cell received as
+   the j cell of other cells' tasks and in its own self task, in a fixed
+   order (the self entry, then the incoming entries in ascending order),
+   all added with two-sum. */
+__global__ __launch_bounds__(64) void k_fold(int ncells, const int2 *__restrict__ cnt_off, int nsplit, int nparts,
+                                             const float4 *__restrict__ phi, const float4 *__restrict__ plo, int half,
                                              const int *__restrict__ self_entry, const int *__restrict__ inv_off,
                                              const int *__restrict__ inv_list, const float4 *__restrict__ jpart,
                                              int jcap, float4 *__restrict__ fhi, float4 *__restrict__ flo) {
     const int c = blockIdx.x, lane = threadIdx.x;
     const int2 co = cnt_off[c];
-    const int q0 = inv_off[c], q1 = inv_off[c + 1], es = self_entry[c];
+    const int q0 = half ? inv_off[c] : 0, q1 = half ? inv_off[c + 1] : 0, es = half ? self_entry[c] : -1;
     for (int s = lane; s < co.x; s += 64) {
         const int a = co.y + s;
-        const float4 h = fhi[a], l = flo[a];
+        const float4 h = nsplit > 1 ? phi[a] : fhi[a], l = nsplit > 1 ? plo[a] : flo[a];
         float hx = h.x, hy = h.y, hz = h.z, lx = l.x, ly = l.y, lz = l.z;
+        for (int gp = 1; gp < nsplit; gp++) {
+            const float4 oh = phi[(size_t)gp * nparts + a], ol = plo[(size_t)gp * nparts + a];
+            acc2(hx, lx, oh.x);
+            acc2(hy, ly, oh.y);
+            acc2(hz, lz, oh.z);
+            lx += ol.x;
+            ly += ol.y;
+            lz += ol.z;
+        }
         if (es >= 0) {
             const float4 v = jpart[(size_t)es * jcap + s];
             acc2(hx, lx, v.x);
@@ -1299,6 +1325,8 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_inv_off);
     (void)hipFree(g->d_inv_list);
     (void)hipFree(g->d_jpart);
+    (void)hipFree(g->d_phi);
+    (void)hipFree(g->d_plo);
     (void)hipFree(g->d_origin);
     (void)hipFree(g->d_loc);
     (void)hipFree(g->d_sortflags);
@@ -1698,9 +1726,25 @@ extern "C" int engine_gpu_load(struct engine *e) {
         if (gpu_alloc_ids(g, nids > 0 ? nids : 1) < 0) return engine_err_gpu;
     }
 
+    {
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, g->device));
+        g->ncu = prop.multiProcessorCount;
+    }
+    /* Few, densely filled cells: split each owner's neighbour list into
+       nsplit groups, so that the pair kernel has ~MD_SPLIT_ITEMS_PER_CU work
+       items per CU.  (With few atoms per cell a visit is short and the extra
+       items cost more than they spread.) */
+    g->nsplit = 1;
+    if (!(e->flags & engine_flag_nosplit) && s->nr_cells < MD_SPLIT_ITEMS_PER_CU * g->ncu &&
+        (long long)s->nr_parts >= (long long)engine_gpu_half_mean * s->nr_cells) {
+        g->nsplit = (MD_SPLIT_ITEMS_PER_CU * g->ncu + s->nr_cells - 1) / s->nr_cells;
+        if (g->nsplit > MD_SPLIT_MAX) g->nsplit = MD_SPLIT_MAX;
+    }
+    g->P.nsplit = g->nsplit;
     HIPCHECK(hipMalloc(&g->d_cnt_off, sizeof(int2) * s->nr_cells));
-    HIPCHECK(hipMalloc(&g->d_cell_acc, sizeof(double) * 3 * s->nr_cells));
-    HIPCHECK(hipMalloc(&g->d_cell_cnt, sizeof(unsigned long long) * 3 * s->nr_cells));
+    HIPCHECK(hipMalloc(&g->d_cell_acc, sizeof(double) * 3 * (size_t)s->nr_cells * g->nsplit));
+    HIPCHECK(hipMalloc(&g->d_cell_cnt, sizeof(unsigned long long) * 3 * (size_t)s->nr_cells * g->nsplit));
     HIPCHECK(hipMalloc(&g->d_tot, sizeof(double) * 4));
     HIPCHECK(hipMalloc(&g->d_tot_cnt, sizeof(unsigned long long) * 4));
     HIPCHECK(hipMalloc(&g->d_queue, sizeof(int) * (MD_NQUEUE + 1)));
@@ -1715,12 +1759,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
     HIPCHECK(hipHostMalloc((void **)&g->h_tot, sizeof(double) * 4, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_tot_cnt, sizeof(unsigned long long) * 4, hipHostMallocDefault));
 
-    {
-        hipDeviceProp_t prop;
-        HIPCHECK(hipGetDeviceProperties(&prop, g->device));
-        g->ncu = prop.multiProcessorCount;
-    }
-    g->lds_nb[0] = g->lds_nb[1] = g->lds_nb[2] = 0;
+    for (int k = 0; k < 5; k++) g->lds_nb[k] = 0;
     return engine_err_ok;
 }
 
@@ -1816,6 +1855,18 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
         g->jcap = (int)maxn + 8;
         HIPCHECK(hipMalloc(&g->d_jpart, sizeof(float4) * (size_t)g->jcap * (g->n_entries_h > 0 ? g->n_entries_h : 1)));
     }
+    if (g->nsplit > 1 && g->pcap < (size_t)g->nsplit * g->nparts) {
+        /* per neighbour group, the owner forces of every atom; no pending
+           pass may still use the old buffers */
+        HIPCHECK(hipStreamSynchronize(g->stream));
+        (void)hipFree(g->d_phi);
+        (void)hipFree(g->d_plo);
+        g->d_phi = g->d_plo = NULL;
+        g->pcap = 0;
+        HIPCHECK(hipMalloc(&g->d_phi, sizeof(float4) * (size_t)g->nsplit * g->nparts));
+        HIPCHECK(hipMalloc(&g->d_plo, sizeof(float4) * (size_t)g->nsplit * g->nparts));
+        g->pcap = (size_t)g->nsplit * g->nparts;
+    }
     if (lds != g->lds_nb[kern]) {
         /* Persistent grid: as many single-wave blocks as can be resident. */
         int per_cu = 0;
@@ -1867,28 +1918,37 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     O.cell_cnt = g->d_cell_cnt;
     O.jpart = g->d_jpart;
     O.jcap = g->jcap;
+    O.phi = g->d_phi;
+    O.plo = g->d_plo;
+    /* no more waves than work items: the dispatcher then spreads them over
+       all compute units */
+    const int nitems = g->ncells * g->nsplit;
+    const int grid = g->grid_nb[kern] < nitems ? g->grid_nb[kern] : nitems;
     if (kern >= 3) {
-        hipLaunchKernelGGL(kern == 4 ? k_nonbond_half_wide : k_nonbond_half, dim3(g->grid_nb[kern]), dim3(64), lds,
+        hipLaunchKernelGGL(kern == 4 ? k_nonbond_half_wide : k_nonbond_half, dim3(grid), dim3(64), lds,
                            g->stream, P, g->d_cnt_off, g->d_nbr_h, g->d_nbr_off_h, g->d_nbs_h, g->d_origin, g->d_loc,
                            g->d_xf, g->d_x, g->d_id, g->d_sortlist, g->d_keyof, g->d_ptab,
                            (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list, g->d_index_of_id,
                            g->d_cell_of, O, g->d_queue);
-        hipLaunchKernelGGL(k_fold, dim3(g->ncells), dim3(64), 0, g->stream, g->ncells, g->d_cnt_off, g->d_self_entry,
-                           g->d_inv_off, g->d_inv_list, (const float4 *)g->d_jpart, g->jcap, g->d_fhi, g->d_flo);
     } else if (kern == 1)
-        hipLaunchKernelGGL(k_nonbond_mlp, dim3(g->grid_nb[1]), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
+        hipLaunchKernelGGL(k_nonbond_mlp, dim3(grid), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue, gpu_mlp_net(g), g->d_ptype);
     else
-        hipLaunchKernelGGL(kern == 2 ? k_nonbond_wide : k_nonbond, dim3(g->grid_nb[kern]), dim3(64), lds, g->stream,
+        hipLaunchKernelGGL(kern == 2 ? k_nonbond_wide : k_nonbond, dim3(grid), dim3(64), lds, g->stream,
                            P, g->d_cnt_off, g->d_nbr,
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue);
+    if (kern >= 3 || g->nsplit > 1)
+        hipLaunchKernelGGL(k_fold, dim3(g->ncells), dim3(64), 0, g->stream, g->ncells, g->d_cnt_off, g->nsplit,
+                           g->nparts, (const float4 *)g->d_phi, (const float4 *)g->d_plo, kern >= 3 ? 1 : 0,
+                           g->d_self_entry, g->d_inv_off, g->d_inv_list, (const float4 *)g->d_jpart, g->jcap,
+                           g->d_fhi, g->d_flo);
     if (timed) HIPCHECK(hipEventRecord(g->ev[2], g->stream));
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, g->stream, g->ncells, g->d_cell_acc, g->d_cell_cnt, g->d_tot,
-                       g->d_tot_cnt);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, g->stream, g->ncells * g->nsplit, g->d_cell_acc,
+                       g->d_cell_cnt, g->d_tot, g->d_tot_cnt);
     if (timed) HIPCHECK(hipEventRecord(g->ev[3], g->stream));
     HIPCHECK(hipGetLastError());
     return engine_err_ok;

@@ -42,6 +42,7 @@ struct nb_params {
     int ptab_bytes, coef_bytes; /* sizes of the table headers and coefficients */
     /* dynamic LDS layout (byte offsets) */
     int lds_own_key, lds_kfl, lds_stage, lds_hit, lds_dense, lds_jf;
+    int nsplit; /* neighbour groups per owner cell (work items per cell) */
 };
 
 /* Neighbour-list entry of an owner cell: x = other cell, y = sid | orient<<4
@@ -76,6 +77,11 @@ struct gpu_state {
     int *d_nbr_off_h, *d_self_entry, *d_inv_off, *d_inv_list;
     int n_entries_h, jcap, half;
     float4 *d_jpart;
+    /* split owners: nsplit neighbour groups per cell; per group the owner
+       forces of every atom (pcap float4 allocated in each) */
+    int nsplit;
+    size_t pcap;
+    float4 *d_phi, *d_plo;
     double *d_origin;
     int *d_loc;
     unsigned int *d_sortflags;

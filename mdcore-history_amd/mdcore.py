@@ -33,6 +33,7 @@ engine_flag_mlp = 131072
 engine_flag_fullshell = 262144
 engine_flag_halfshell = 524288
 engine_gpu_half_mean = 48
+engine_flag_nosplit = 1048576
 
 space_periodic_full = 7
 

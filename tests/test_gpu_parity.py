@@ -5,7 +5,9 @@ exclusion counts.  Floating point: per-atom force relative RMSE <= 1e-5 and
 energy / virial relative error <= 1e-6 against the FP64 oracle (FP32 pair
 arithmetic, two-sum accumulation, FP64 reductions).  Every system runs with
 both pair-kernel shells (full: each pair at both cells; half: each pair task
-once, opposite forces folded per atom).
+once, opposite forces folded per atom), with each owner cell's neighbours
+split over several work items (the default for these small systems) and
+with one item per cell.
 """
 import numpy as np
 import pytest
@@ -61,16 +63,18 @@ def dense_big_cells(L=3.9, density=150.0, seed=3):
 
 
 SHELLS = {"full": md.engine_flag_fullshell, "half": md.engine_flag_halfshell}
+SPLITS = {"split": 0, "nosplit": md.engine_flag_nosplit}
+MODES = {f"{sh}-{sp}": SHELLS[sh] | SPLITS[sp] for sh in sorted(SHELLS) for sp in sorted(SPLITS)}
 
 
-@pytest.fixture(scope="module", params=[(s, sh) for s in sorted(SYSTEMS) for sh in sorted(SHELLS)],
+@pytest.fixture(scope="module", params=[(s, m) for s in sorted(SYSTEMS) for m in MODES],
                 ids=lambda p: f"{p[0]}-{p[1]}")
 def case(request):
-    name, shell = request.param
+    name, mode = request.param
     sys = SYSTEMS[name]()
     orc = po.from_system(sys)
-    eng = engine_pass(sys, md.engine_flag_gpu | SHELLS[shell])
-    yield f"{name}-{shell}", sys, orc, eng
+    eng = engine_pass(sys, md.engine_flag_gpu | MODES[mode])
+    yield f"{name}-{mode}", sys, orc, eng
     eng.close()
     orc.close()
 
@@ -123,8 +127,8 @@ def test_forces_energy_vs_fp64(case):
 
 def test_projected_window_finds_every_pair(case):
     name, sys, orc, _ = case
-    shell = SHELLS[name.rsplit("-", 1)[1]]
-    eng = engine_pass(sys, md.engine_flag_gpu | md.engine_flag_projwindow | shell)
+    mode = MODES[name.split("-", 1)[1]]
+    eng = engine_pass(sys, md.engine_flag_gpu | md.engine_flag_projwindow | mode)
     try:
         g = eng.gpu_results()
         p = orc.nonbond(po.OR_WINDOW_PROJ)
