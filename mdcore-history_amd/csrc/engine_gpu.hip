@@ -347,17 +347,154 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
     sort_desc(list, n, flag);
 }
 
+/* The quicksort of sort_desc_t, run by a whole wave on one LDS list with the
+   same result, element for element.
+   - Segments below 16 elements (selection sort): each step takes the first
+     maximal key of the rest, a wave-wide max of key << 6 | (63 - index), and
+     swaps it to the front.
+   - Partition (pivot p = middle key): the serial scan's k-th swap pairs the
+     k-th element from the left with key <= p (ltab[k]) and the k-th from the
+     right with key >= p (rtab[k]), for k = 1..K while ltab[k] <= rtab[k];
+     both tables come from ballots and prefix counts.  The scan then stops
+     with i = ltab[K] + 1, j = rtab[K] - 1 if these have crossed, else at
+     i = min(ltab[K+1], rtab[K]), j = max(rtab[K+1], ltab[K]) (the swapped
+     elements stop the pointers), without a further swap.
+   The stack (same pushes and overflow test as sort_desc_t) lives in LDS and
+   is written by lane 0.  ltab / rtab: maxn + 1 entries each. */
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned int u = (unsigned int)__shfl_xor((int)v, o);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+__device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, unsigned short *ltab,
+                               unsigned short *rtab, int *overflow, int lane) {
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int q = 0;
+    if (lane == 0) {
+        st_lo[0] = 0;
+        st_hi[0] = N - 1;
+    }
+    wave_lds_sync();
+    int pops = 0;
+    while (q >= 0) {
+        const int lo = st_lo[q], hi = st_hi[q];
+        q -= 1;
+        /* the serial sort pops at most 2N segments, all inside [0, N): anything
+           else is reported as a sort failure instead of looping on */
+        if (++pops > 2 * N + 2 || lo < 0 || hi >= N || lo > hi) {
+            if (lane == 0) *overflow = 1;
+            return;
+        }
+        if (hi - lo < 15) {
+            const int len = hi - lo + 1;
+            unsigned int w = lane < len ? a[lo + lane] : 0u;
+            for (int i = 0; i < len - 1; i++) {
+                const unsigned int pk =
+                    (lane >= i && lane < len) ? (((w & 0xffffu) << 6) | (unsigned int)(63 - lane)) : 0u;
+                const int im = 63 - (int)(wave_max_u32(pk) & 63u);
+                const unsigned int wi = (unsigned int)__shfl((int)w, i), wm = (unsigned int)__shfl((int)w, im);
+                if (lane == i)
+                    w = wm;
+                else if (lane == im)
+                    w = wi;
+            }
+            if (lane < len) a[lo + lane] = w;
+            wave_lds_sync();
+            continue;
+        }
+        const unsigned int p = a[(lo + hi) / 2] & 0xffffu;
+        unsigned int wv[3];
+        bool isl[3], isr[3];
+        int nR = 0;
+        for (int r = 0; r < 3; r++) {
+            const int x = lo + lane + 64 * r;
+            const bool valid = x <= hi;
+            wv[r] = valid ? a[x] : 0u;
+            isl[r] = valid && (wv[r] & 0xffffu) <= p;
+            isr[r] = valid && (wv[r] & 0xffffu) >= p;
+            nR += __builtin_popcountll(__builtin_amdgcn_ballot_w64(isr[r]));
+        }
+        int cl = 0, cr = 0;
+        for (int r = 0; r < 3; r++) {
+            const int x = lo + lane + 64 * r;
+            const unsigned long long ml = __builtin_amdgcn_ballot_w64(isl[r]);
+            const unsigned long long mr = __builtin_amdgcn_ballot_w64(isr[r]);
+            if (isl[r]) ltab[cl + __builtin_popcountll(ml & below) + 1] = (unsigned short)x;
+            if (isr[r]) rtab[nR - cr - __builtin_popcountll(mr & below)] = (unsigned short)x;
+            cl += __builtin_popcountll(ml);
+            cr += __builtin_popcountll(mr);
+        }
+        const int nL = cl, m = nL < nR ? nL : nR;
+        wave_lds_sync();
+        int K = 0;
+        int li[3], ri[3];
+        for (int r = 0; r < 3; r++) {
+            const int k = 1 + lane + 64 * r;
+            li[r] = k <= m ? ltab[k] : 0;
+            ri[r] = k <= m ? rtab[k] : 0;
+            K += __builtin_popcountll(__builtin_amdgcn_ballot_w64(k <= m && li[r] <= ri[r]));
+        }
+        /* the K swaps touch distinct elements, each lane its own */
+        for (int r = 0; r < 3; r++) {
+            const int k = 1 + lane + 64 * r;
+            if (k <= K && li[r] < ri[r]) {
+                const unsigned int va = a[li[r]], vb = a[ri[r]];
+                a[li[r]] = vb;
+                a[ri[r]] = va;
+            }
+        }
+        wave_lds_sync();
+        const int lK = ltab[K], rK = rtab[K];
+        int i = lK + 1, j = rK - 1;
+        if (i <= j) {
+            const int l1 = K + 1 <= nL ? (int)ltab[K + 1] : 1 << 30;
+            const int r1 = K + 1 <= nR ? (int)rtab[K + 1] : -(1 << 30);
+            i = l1 < rK ? l1 : rK;
+            j = r1 > lK ? r1 : lK;
+        }
+        if (q + 2 >= MD_SORT_STACK) {
+            if (lane == 0) *overflow = 1;
+            return;
+        }
+        const int mid = (lo + hi) / 2;
+        const bool left_first = j > mid;
+        /* pushes in sort_desc_t's order: [lo, j] then [i, hi] when j > mid */
+        const bool pa = left_first ? lo < j : i < hi, pb = left_first ? i < hi : lo < j;
+        const int alo = left_first ? lo : i, ahi = left_first ? j : hi;
+        const int blo = left_first ? i : lo, bhi = left_first ? hi : j;
+        if (pa) {
+            q++;
+            if (lane == 0) {
+                st_lo[q] = alo;
+                st_hi[q] = ahi;
+            }
+        }
+        if (pb) {
+            q++;
+            if (lane == 0) {
+                st_lo[q] = blo;
+                st_hi[q] = bhi;
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
 /* Sort by rank: one workgroup per cell, all 13 directions at once.  A key that no
    other particle of its list shares has exactly (number of greater keys)
    particles before it in any descending order, the reference's quicksort
    included, so those particles are placed by counting.  Only the order of
    equal keys depends on the quicksort's partitions: a list with equal keys
-   is sorted again by one lane with the reference's algorithm
-   (runner.c:110-174), in LDS, and written whole.  For cells of at most
-   MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions, 13 * maxn list
-   words, 13 quicksort stacks. */
+   is sorted again with the reference's algorithm (runner.c:110-174), in LDS,
+   and written whole: by a whole wave (sort_desc_wave) when the cell has a
+   wave per direction, else by one lane.  For cells of at
+   most MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions, 13 * maxn list
+   words, 13 quicksort stacks, 13 pairs of partition tables. */
 #define MD_SORT_LDS_MAX 191
-__global__ __launch_bounds__(256) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
+__global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
                                                   const int *__restrict__ offset,
                                                   const unsigned int *__restrict__ sortflags,
                                                   const double *__restrict__ x, unsigned int *__restrict__ sortlist,
@@ -367,6 +504,7 @@ __global__ __launch_bounds__(256) void k_sort_rank(struct nb_params P, const int
     double *xs = sh_rank;
     unsigned int *lists = (unsigned int *)(sh_rank + 3 * maxn);
     int *stk = (int *)(lists + 13 * maxn);
+    unsigned short *tabs = (unsigned short *)(stk + 13 * 2 * MD_SORT_STACK);
     __shared__ int tie[13];
     const int c = blockIdx.x, lane = threadIdx.x;
     const int n = count[c];
@@ -402,7 +540,20 @@ __global__ __launch_bounds__(256) void k_sort_rank(struct nb_params P, const int
             tie[sid] = 1;
     }
     __syncthreads();
-    if (lane < 13 && ((fl >> lane) & 1u) && tie[lane]) {
+    /* Lists with a repeated key: the reference's quicksort.  With 13 waves
+       per cell, wave l sorts list l cooperatively (sort_desc_wave); with one
+       wave per cell (small cells, few such lists), lane l sorts list l. */
+    const int wave = lane >> 6, wl = lane & 63;
+    if (blockDim.x >= 13 * 64) {
+        const int l = wave;
+        if (l < 13 && ((fl >> l) & 1u) && tie[l]) {
+            unsigned int *L = lists + l * n;
+            int *st = stk + l * 2 * MD_SORT_STACK;
+            unsigned short *tb = tabs + (size_t)l * 2 * (maxn + 1);
+            sort_desc_wave(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
+            for (int i = wl; i < n; i += 64) out[l * n + i] = L[i];
+        }
+    } else if (lane < 13 && ((fl >> lane) & 1u) && tie[lane]) {
         unsigned int *L = lists + lane * n;
         int *st = stk + lane * 2 * MD_SORT_STACK;
         sort_desc_t(L, n, st, st + MD_SORT_STACK, flag);
@@ -1899,9 +2050,10 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     } else if (g->maxcount <= MD_SORT_LDS_MAX) {
         const int maxn = g->maxcount > 0 ? g->maxcount : 1;
         const size_t sh = sizeof(double) * 3 * maxn + sizeof(unsigned int) * 13 * maxn +
-                          sizeof(int) * 13 * 2 * MD_SORT_STACK;
-        /* one wave per cell; four for cells large enough to keep them busy */
-        const int bs = maxn > 48 ? 256 : 64;
+                          sizeof(int) * 13 * 2 * MD_SORT_STACK + sizeof(unsigned short) * 13 * 2 * (maxn + 1);
+        /* one wave per cell; for large cells (frequent repeated keys) one
+           wave per direction */
+        const int bs = maxn > 48 ? 13 * 64 : 64;
         hipLaunchKernelGGL(k_sort_rank, dim3(g->ncells), dim3(bs), sh, g->stream, P, g->d_count, g->d_offset,
                            g->d_sortflags, g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE, maxn);
     } else {

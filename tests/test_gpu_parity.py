@@ -36,6 +36,9 @@ SYSTEMS = {
     # (<= 256 particles per cell), then to 10 (<= 64 per cell)
     "many_types_100": lambda: many_types(100),
     "many_types_600": lambda: many_types(600),
+    # simple cubic lattice, 64 atoms per cell: every direction's keys repeat
+    # many times, so nearly every sort list goes through the tie sort
+    "lattice_ties": lambda: lattice_ties(),
 }
 
 
@@ -52,6 +55,19 @@ def many_types(ntypes, N=4000, nbase=5, seed=13):
             for j in range(i, ntypes + 1)]
     masses = np.concatenate([[0.0], [base.masses[1 + (t % nbase)] for t in range(ntypes)]])
     return systems.System("many_types", base.origin, base.dim, base.L, base.cutoff, base.x, types, masses, pots)
+
+
+def lattice_ties(n=16, a=0.25, seed=17):
+    """Two species at random on the sites, so that the net forces do not
+    cancel by symmetry."""
+    g = (np.arange(n) + 0.5) * a
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    L = n * a
+    types = md.gen_types(len(x), 2, 1, seed)
+    pots = [(1, 1, md.LJ126(0.05, 1.0, 1e-6, 1e-3, 1e-3)), (1, 2, md.LJ126(0.05, 1.0, 2e-6, 1.5e-3, 1e-3)),
+            (2, 2, md.LJ126(0.05, 1.0, 4e-6, 2.5e-3, 1e-3))]
+    return systems.System("lattice", np.zeros(3), np.full(3, L), np.full(3, 1.0), 1.0, x, types,
+                          np.array([0.0, 1.0, 2.0]), pots)
 
 
 def dense_big_cells(L=3.9, density=150.0, seed=3):
