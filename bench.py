@@ -22,6 +22,7 @@ passes timed with HIP events on the engine's stream.  Prints one JSON line.
   nve           energy drift of 1,000 NVE steps (dt 2 fs) of the 32,000-atom
                 argon system with the table pass and with a trained network,
                 and of the 100k-atom rigid water on the device-resident step
+                (3 ps rescaled to 300 K, then 2,000 NVE steps of 1 fs)
   md_step       whole MD steps (shuffle, pass, kick-drift, migration) of the
                 1,048,576-atom argon crystal at 100 K, dt 2 fs: host-driven
                 engine_step (positions up, forces down, host integration)
@@ -156,7 +157,7 @@ def main():
             traffic = None
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_nonbond_wide" if eng_cells(sysm) >= 64 * MI355X_CUS else "k_nonbond",
+                "kernel": pair_kernel_name(sysm, md),
                 "kernel_ms": round(ms["nonbond_ms"], 4),
                 "note": "FP32 vector pipe (VALU); packed FP32 disabled, so 78.65 TFLOP/s is the issue ceiling"}
 
@@ -335,7 +336,7 @@ def nve_section(steps=1000, dt=0.002, equil=1000):
     t2 = time.perf_counter()
     rt, at = sg.drift_per_ns(e_tab, dt)
     rm, am = sg.drift_per_ns(e_mlp, dt)
-    water = water_nve(steps, dt, equil)
+    water = water_nve()
     return {"system": "argon_32k: 32,000 LJ atoms, fcc start at 100 K, rc 1.0 nm", "steps": steps, "dt_ps": dt,
             "equilibration_steps": equil, "water_rigid": water,
             "table": {"drift_rel_per_ns": rt, "drift_kJ_mol_per_ns": at, "e_mean": float(e_tab.mean()),
@@ -411,38 +412,43 @@ def md_step_run(nsteps, host_steps, dt, cell, verlet):
     return out
 
 
-def water_nve(steps, dt, equil):
+def water_nve(dt=0.001, equil=3000, steps=2000, rescale_every=25, T=300.0):
     """Rigid water (P-SHAKE on the device) through engine_gpu_steps: 33,333
-    molecules, 0.5 nm cells with the projected window (the reference window
-    misses in-cutoff pairs on half-cutoff cells), Maxwell start at 300 K
-    from the lattice; energy drift after equilibration and the largest
-    constraint residue at the end."""
+    molecules, 1.0 nm cells with the reference window.  The lattice start releases
+    enough energy to heat the box to ~1100 K, so `equil` steps rescale the
+    velocities to T every `rescale_every` steps (engine_setvelocities, as the
+    reference's drivers rescale p->v); then `steps` NVE steps give the energy
+    drift, and the largest constraint residue is taken at the end."""
     import dataclasses
     import numpy as np
     import mdcore as md
     import surrogate as sg
     import systems
-    sysm = systems.water(rigid=True, cell=0.5)
-    sysm = dataclasses.replace(sysm, v=md.gen_maxwell(sysm.masses[sysm.types], systems.KB * 300.0, 11))
-    eng = systems.build_engine(sysm, md.engine_flag_gpu | md.engine_flag_projwindow)
+    sysm = systems.water(rigid=True, cell=1.0)
+    sysm = dataclasses.replace(sysm, v=md.gen_maxwell(sysm.masses[sysm.types], systems.KB * T, 11))
+    c = np.array(sysm.rigid)
+    dof = 3 * sysm.N - len(c) - 3
+    eng = systems.build_engine(sysm, md.engine_flag_gpu)
     eng.setdt(dt)
     t0 = time.perf_counter()
-    eng.gpu_steps(equil)
+    for _ in range(equil // rescale_every):
+        _, ek = eng.gpu_steps(rescale_every)
+        eng.set_velocities(eng.velocities() * np.sqrt(T / (2 * ek[-1] / (dof * systems.KB))))
     ep, ek = eng.gpu_steps(steps)
     wall = time.perf_counter() - t0
     x = eng.positions()
     eng.close()
-    c = np.array(sysm.rigid)
     i, j, d = c[:, 0].astype(int), c[:, 1].astype(int), c[:, 2]
     dv = x[i] - x[j]
     dv -= sysm.dim * np.round(dv / sysm.dim)
     res = np.abs(d * d - (dv * dv).sum(1)).max()
     et = ep + ek
     rel, absd = sg.drift_per_ns(et, dt)
-    dof = 3 * sysm.N - len(c) - 3
     return {"system": "water_100k rigid: 33,333 molecules (99,999 atoms), 10 nm box, LJ+RF, rc 1.0 nm, "
-                      "0.5 nm cells, projected window, P-SHAKE",
-            "drift_rel_per_ns": rel, "drift_kJ_mol_per_ns": absd, "e_mean": float(et.mean()), "e_std": float(et.std()),
+                      "1.0 nm cells, reference window, P-SHAKE",
+            "dt_ps": dt, "equilibration_steps": equil, "rescale_every": rescale_every, "steps": steps,
+            "drift_rel_per_ns": rel, "drift_kJ_mol_per_ns": absd, "drift_kJ_mol_per_ns_per_dof": absd / dof,
+            "e_mean": float(et.mean()), "e_std": float(et.std()),
             "temperature_K": float(2 * ek.mean() / (dof * systems.KB)),
             "max_constraint_residue_nm2": float(res), "ms_per_step": round(wall / (equil + steps) * 1e3, 3)}
 
@@ -463,6 +469,17 @@ def copy_calibration(mib):
 def eng_cells(sysm):
     import numpy as np
     return int(np.prod(np.floor(sysm.dim / sysm.L)))
+
+
+def pair_kernel_name(sysm, md):
+    """The pair kernel engine_gpu_load picks: the half shell (with k_fold) for
+    engine_gpu_half_mean or more atoms per cell on average, the 5-wave build
+    from 64 cells per CU."""
+    cells = eng_cells(sysm)
+    wide = cells >= 64 * MI355X_CUS
+    if sysm.N >= md.engine_gpu_half_mean * cells:
+        return ("k_nonbond_half_wide" if wide else "k_nonbond_half") + " + k_fold"
+    return "k_nonbond_wide" if wide else "k_nonbond"
 
 
 if __name__ == "__main__":

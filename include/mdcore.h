@@ -546,6 +546,10 @@ int engine_getforces(struct engine *e, double *f, int N);
 int engine_getpositions(struct engine *e, double *x, int N);
 /* Velocities v[3*id+k], by particle id. */
 int engine_getvelocities(struct engine *e, double *v, int N);
+/* Set the velocities from v[3*id+k], by particle id (the reference's drivers
+   assign p->v directly, e.g. to rescale to a temperature); engine_gpu_steps
+   takes them at its next call. */
+int engine_setvelocities(struct engine *e, const double *v, int N);
 /* energies[0..3] = epot, epot_nonbond, epot_exclusion, virial; counts[0..1] = pairs, exclusions. */
 int engine_getenergies(struct engine *e, double *energies, long long *counts);
 /* Copy the host sortlist of cell cid, direction sid, into out (count entries). */

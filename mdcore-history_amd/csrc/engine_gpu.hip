@@ -543,10 +543,10 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
     /* Lists with a repeated key: the reference's quicksort.  With 13 waves
        per cell, wave l sorts list l cooperatively (sort_desc_wave); with one
        wave per cell (small cells, few such lists), lane l sorts list l. */
-    const int wave = lane >> 6, wl = lane & 63;
-    if (blockDim.x >= 13 * 64) {
-        const int l = wave;
-        if (l < 13 && ((fl >> l) & 1u) && tie[l]) {
+    const int wave = lane >> 6, nwave = (int)blockDim.x >> 6, wl = lane & 63;
+    if (nwave > 1) {
+        for (int l = wave; l < 13; l += nwave) {
+            if (!((fl >> l) & 1u) || !tie[l]) continue;
             unsigned int *L = lists + l * n;
             int *st = stk + l * 2 * MD_SORT_STACK;
             unsigned short *tb = tabs + (size_t)l * 2 * (maxn + 1);

@@ -85,6 +85,7 @@ def library():
         "engine_getforces": (C.c_int, [C.c_void_p, _dp, C.c_int]),
         "engine_getpositions": (C.c_int, [C.c_void_p, _dp, C.c_int]),
         "engine_getvelocities": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+        "engine_setvelocities": (C.c_int, [C.c_void_p, _dp, C.c_int]),
         "engine_getenergies": (C.c_int, [C.c_void_p, _dp, _llp]),
         "engine_getsortlist": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up]),
         "engine_getcells": (C.c_int, [C.c_void_p, _ip, _ip]),
@@ -324,6 +325,12 @@ class Engine:
         v = np.zeros(3 * self.N)
         check(self.lib.engine_getvelocities(self.e, ptr(v), self.N), "engine_getvelocities")
         return v.reshape(-1, 3)
+
+    def set_velocities(self, v):
+        vv = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        if vv.size != 3 * self.N:
+            raise ValueError("set_velocities: need 3 values per particle")
+        check(self.lib.engine_setvelocities(self.e, ptr(vv), self.N), "engine_setvelocities")
 
     def energies(self):
         en = np.zeros(4)

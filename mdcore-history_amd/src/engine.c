@@ -498,6 +498,17 @@ int engine_getvelocities(struct engine *e, double *v, int N) {
     return engine_err_ok;
 }
 
+int engine_setvelocities(struct engine *e, const double *v, int N) {
+    int id, k;
+    if (e == NULL || v == NULL) return eerror_(engine_err_null);
+    for (id = 0; id < N; id++) {
+        struct part *p = (id < e->s.size_parts) ? e->s.partlist[id] : NULL;
+        if (p == NULL) return eerror_(engine_err_range);
+        for (k = 0; k < 3; k++) p->v[k] = (FPTYPE)v[3 * id + k];
+    }
+    return engine_err_ok;
+}
+
 int engine_getenergies(struct engine *e, double *energies, long long *counts) {
     if (e == NULL) return eerror_(engine_err_null);
     if (energies) {

@@ -149,3 +149,17 @@ def test_sample_pairs_finds_every_pair_once_per_side():
 def test_sample_pairs_rejects_small_box():
     with pytest.raises(md.MdcoreError):
         md.sample_pairs(np.zeros((4, 3)), np.ones(4, np.int32), [2.0] * 3, 1.0, 10, 1)
+
+
+def test_set_velocities_round_trip():
+    """engine_setvelocities writes p->v by particle id; engine_getvelocities
+    reads the same values back, after a shuffle has reordered the cells."""
+    sysm = systems.argon(4, jitter=0.05)
+    eng = systems.build_engine(sysm, md.engine_flag_gpu, start=False)
+    eng.shuffle()
+    v = md.gen_uniform(3 * sysm.N, -1.0, 1.0, 5).reshape(-1, 3)
+    eng.set_velocities(v)
+    assert np.array_equal(eng.velocities(), v)
+    with pytest.raises(ValueError):
+        eng.set_velocities(v[:-1])
+    eng.close()
