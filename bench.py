@@ -19,10 +19,11 @@ passes timed with HIP events on the engine's stream.  Prints one JSON line.
                 trained on the GPU, timed like the table pass, its force
                 error against FP64, and a table set rebuilt at the tolerance
                 whose force error matches the network's, timed the same way
-  nve           energy drift of 1,000 NVE steps (dt 2 fs) of the 32,000-atom
-                argon system with the table pass and with a trained network,
-                and of the 100k-atom rigid water on the device-resident step
-                (3 ps rescaled to 300 K, then 2,000 NVE steps of 1 fs)
+  nve           energy drift of 10,000 device-resident NVE steps (dt 2 fs)
+                of the 32,000-atom argon system: the plain LJ table, the
+                force-continuous switched table, and a trained network on
+                each; and of the 100k-atom rigid water (3 ps rescaled to
+                300 K, then 2,000 NVE steps of 1 fs)
   md_step       whole MD steps (shuffle, pass, kick-drift, migration) of the
                 1,048,576-atom argon crystal at 100 K, dt 2 fs: host-driven
                 engine_step (positions up, forces down, host integration)
@@ -305,44 +306,62 @@ def matched_table(sysm, f_table, target, args):
             "unit": "pair-interactions/s", "ms_per_step": ms["total_ms"]}
 
 
-def nve_section(steps=1000, dt=0.002, equil=1000):
-    """Energy drift of the 32,000-atom argon system over `steps` NVE steps
-    after `equil` steps away from the lattice start, table pass vs
-    surrogate pass (network trained on the table's trajectory)."""
+def nve_section(steps=10000, dt=0.002, equil=1000):
+    """Energy drift of the 32,000-atom argon system (fcc start at 100 K) over
+    `steps` device-resident NVE steps after `equil` steps: the configuration's
+    plain LJ 12-6 table (not shifted: pairs crossing the cutoff jump in
+    energy), the force-continuous LJ126_switch table (switch from 0.9 nm),
+    and the surrogate pass trained on each table over the equilibrated
+    configuration.  Total energy at step n: PE(x_n) + (KE(v_{n-1/2}) +
+    KE(v_{n+1/2})) / 2; drift = least-squares slope, per ns, relative to
+    |<E>| and in kJ/mol."""
+    import dataclasses
+    import mdcore as md
+    import systems
+    base = systems.argon(20)
+    m = base.meta
+    switched = dataclasses.replace(
+        base, pots=[(1, 1, md.LJ126_switch(0.275, 1.0, m["A"], m["B"], 0.9, 1e-3))])
+    out = {"system": "argon_32k: 32,000 LJ atoms, fcc start at 100 K, rc 1.0 nm", "steps": steps, "dt_ps": dt,
+           "equilibration_steps": equil}
+    for key, sysm, mlp in (("table", base, False), ("mlp", base, True), ("table_switch", switched, False),
+                           ("mlp_switch", switched, True)):
+        out[key] = nve_drift(sysm, mlp, equil, steps, dt)
+    out["water_rigid"] = water_nve()
+    return out
+
+
+def nve_drift(sysm, mlp, equil, steps, dt):
+    import dataclasses
+    import numpy as np
     import mdcore as md
     import surrogate as sg
     import systems
-    sysm = systems.argon(20)
-    masses = sysm.masses[sysm.types]
     t0 = time.perf_counter()
     eng = systems.build_engine(sysm, md.engine_flag_gpu)
-    sg.nve_energies(eng, masses, equil, dt)
-    x_eq = eng.positions()
-    v_eq = eng.velocities()
-    e_tab = sg.nve_energies(eng, masses, steps, dt)
+    eng.setdt(dt)
+    eng.gpu_steps(equil)
+    if mlp:
+        # the surrogate run starts from the same equilibrated state
+        s_eq = dataclasses.replace(sysm, x=eng.positions(), v=eng.velocities())
+        eng.close()
+        eng = systems.build_engine(s_eq, md.engine_flag_gpu)
+        eng.setdt(dt)
+        dom = sg.pair_domains(eng, s_eq)
+        S = sg.make_samples(eng, s_eq, 1 << 20, dom[0], dom[1], configs=[s_eq.x], seed=5)
+        eng.mlp_init(*dom, seed=7)
+        eng.mlp_train(*S, steps=10000, batch_tiles=512, lr0=1e-2, lr1=1e-5, seed=11)
+        eng.setflags(md.engine_flag_mlp, 0)
+    ep, ek = eng.gpu_steps(steps)
     eng.close()
-    t1 = time.perf_counter()
-    # the surrogate run starts from the same equilibrated state
-    s_eq = systems.System(sysm.name, sysm.origin, sysm.dim, sysm.L, sysm.cutoff, x_eq, sysm.types, sysm.masses,
-                          sysm.pots, None, v_eq, sysm.meta)
-    eng = systems.build_engine(s_eq, md.engine_flag_gpu)
-    dom = sg.pair_domains(eng, s_eq)
-    S = sg.make_samples(eng, s_eq, 1 << 20, dom[0], dom[1], configs=[x_eq], seed=5)
-    eng.mlp_init(*dom, seed=7)
-    eng.mlp_train(*S, steps=10000, batch_tiles=512, lr0=1e-2, lr1=1e-5, seed=11)
-    eng.setflags(md.engine_flag_mlp, 0)
-    e_mlp = sg.nve_energies(eng, masses, steps, dt)
-    eng.close()
-    t2 = time.perf_counter()
-    rt, at = sg.drift_per_ns(e_tab, dt)
-    rm, am = sg.drift_per_ns(e_mlp, dt)
-    water = water_nve()
-    return {"system": "argon_32k: 32,000 LJ atoms, fcc start at 100 K, rc 1.0 nm", "steps": steps, "dt_ps": dt,
-            "equilibration_steps": equil, "water_rigid": water,
-            "table": {"drift_rel_per_ns": rt, "drift_kJ_mol_per_ns": at, "e_mean": float(e_tab.mean()),
-                      "e_std": float(e_tab.std()), "wall_s": round(t1 - t0, 1)},
-            "mlp": {"drift_rel_per_ns": rm, "drift_kJ_mol_per_ns": am, "e_mean": float(e_mlp.mean()),
-                    "e_std": float(e_mlp.std()), "wall_s": round(t2 - t1, 1)}}
+    et = ep[1:] + 0.5 * (ek[:-1] + ek[1:])
+    rel, absd = sg.drift_per_ns(et, dt)
+    h = len(et) // 2
+    return {"drift_rel_per_ns": rel, "drift_kJ_mol_per_ns": absd,
+            "drift_rel_per_ns_halves": [sg.drift_per_ns(et[:h], dt)[0], sg.drift_per_ns(et[h:], dt)[0]],
+            "e_mean": float(et.mean()), "e_std": float(et.std()),
+            "temperature_K": float(2 * ek.mean() / (3 * sysm.N * systems.KB)),
+            "wall_s": round(time.perf_counter() - t0, 1)}
 
 
 def md_step_section(nsteps, host_steps=5, dt=0.002):
