@@ -29,7 +29,7 @@
  *               folded into two-sum (hi, lo) accumulators per neighbour.
  *               Excluded pairs are subtracted afterwards as in
  *               exclusion.c:81-282;
- *   k_reduce    per-cell FP64 energies/virials/counts -> totals, fixed order.
+ *   k_reduce_a/b per-item FP64 energies/virials/counts -> totals, fixed order.
  * Every pair is evaluated by both of its cells (once per owner) with
  * bitwise-opposite dx, so the two owners see exactly opposite forces;
  * energy, virial and counts are taken on the task's i side only.
@@ -1338,30 +1338,14 @@ __global__ __launch_bounds__(64) void k_fold(int ncells, const int2 *__restrict_
 
 /* ------------------------------------------------------------ k_reduce */
 
-__global__ __launch_bounds__(1024) void k_reduce(int ncells, const double *__restrict__ cell_acc,
-                                                 const unsigned long long *__restrict__ cell_cnt,
-                                                 double *__restrict__ tot, unsigned long long *__restrict__ tot_cnt) {
-    __shared__ double sd[3][1024];
-    __shared__ unsigned long long sc[3][1024];
-    const int t = threadIdx.x;
-    double d0 = 0, d1 = 0, d2 = 0;
-    unsigned long long c0 = 0, c1 = 0, c2 = 0;
-    for (int k = t; k < ncells; k += 1024) {
-        d0 += cell_acc[3 * k];
-        d1 += cell_acc[3 * k + 1];
-        d2 += cell_acc[3 * k + 2];
-        c0 += cell_cnt[3 * k];
-        c1 += cell_cnt[3 * k + 1];
-        c2 += cell_cnt[3 * k + 2];
-    }
-    sd[0][t] = d0;
-    sd[1][t] = d1;
-    sd[2][t] = d2;
-    sc[0][t] = c0;
-    sc[1][t] = c1;
-    sc[2][t] = c2;
+/* Two steps, both in a fixed order (the totals are bitwise repeatable):
+   k_reduce_a: block b sums the slots [b n / B, (b + 1) n / B) of the
+   per-item FP64 sums and counts (strided per thread, then a tree);
+   k_reduce_b: one block sums the B partials the same way. */
+#define MD_RED_BLOCKS 128
+__device__ __forceinline__ void block_tree3(double (*sd)[256], unsigned long long (*sc)[256], int t) {
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
             for (int m = 0; m < 3; m++) {
                 sd[m][t] += sd[m][t + s];
@@ -1369,11 +1353,59 @@ __global__ __launch_bounds__(1024) void k_reduce(int ncells, const double *__res
             }
         __syncthreads();
     }
-    if (t == 0)
-        for (int m = 0; m < 3; m++) {
-            tot[m] = sd[m][0];
-            tot_cnt[m] = sc[m][0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_a(int n, const double *__restrict__ acc,
+                                                  const unsigned long long *__restrict__ cnt,
+                                                  double *__restrict__ pacc, unsigned long long *__restrict__ pcnt) {
+    __shared__ double sd[3][256];
+    __shared__ unsigned long long sc[3][256];
+    const int t = threadIdx.x, b = blockIdx.x;
+    const int lo = (int)(((long long)b * n) / gridDim.x), hi = (int)(((long long)(b + 1) * n) / gridDim.x);
+    double d0 = 0, d1 = 0, d2 = 0;
+    unsigned long long c0 = 0, c1 = 0, c2 = 0;
+    for (int k = lo + t; k < hi; k += 256) {
+        d0 += acc[3 * k];
+        d1 += acc[3 * k + 1];
+        d2 += acc[3 * k + 2];
+        c0 += cnt[3 * k];
+        c1 += cnt[3 * k + 1];
+        c2 += cnt[3 * k + 2];
+    }
+    sd[0][t] = d0;
+    sd[1][t] = d1;
+    sd[2][t] = d2;
+    sc[0][t] = c0;
+    sc[1][t] = c1;
+    sc[2][t] = c2;
+    block_tree3(sd, sc, t);
+    if (t < 3) {
+        pacc[3 * b + t] = sd[t][0];
+        pcnt[3 * b + t] = sc[t][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_b(int nb, const double *__restrict__ pacc,
+                                                  const unsigned long long *__restrict__ pcnt,
+                                                  double *__restrict__ tot, unsigned long long *__restrict__ tot_cnt) {
+    __shared__ double sd[3][256];
+    __shared__ unsigned long long sc[3][256];
+    const int t = threadIdx.x;
+    for (int m = 0; m < 3; m++) {
+        double d = 0;
+        unsigned long long c = 0;
+        for (int k = t; k < nb; k += 256) {
+            d += pacc[3 * k + m];
+            c += pcnt[3 * k + m];
         }
+        sd[m][t] = d;
+        sc[m][t] = c;
+    }
+    block_tree3(sd, sc, t);
+    if (t < 3) {
+        tot[t] = sd[t][0];
+        tot_cnt[t] = sc[t][0];
+    }
 }
 
 /* ------------------------------------------------------------ host side */
@@ -1488,6 +1520,8 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_cell_acc);
     (void)hipFree(g->d_cell_cnt);
     (void)hipFree(g->d_tot);
+    (void)hipFree(g->d_red_acc);
+    (void)hipFree(g->d_red_cnt);
     (void)hipFree(g->d_tot_cnt);
     (void)hipFree(g->d_queue);
     (void)hipHostFree(g->h_count);
@@ -1897,6 +1931,8 @@ extern "C" int engine_gpu_load(struct engine *e) {
     HIPCHECK(hipMalloc(&g->d_cell_acc, sizeof(double) * 3 * (size_t)s->nr_cells * g->nsplit));
     HIPCHECK(hipMalloc(&g->d_cell_cnt, sizeof(unsigned long long) * 3 * (size_t)s->nr_cells * g->nsplit));
     HIPCHECK(hipMalloc(&g->d_tot, sizeof(double) * 4));
+    HIPCHECK(hipMalloc(&g->d_red_acc, sizeof(double) * 3 * MD_RED_BLOCKS));
+    HIPCHECK(hipMalloc(&g->d_red_cnt, sizeof(unsigned long long) * 3 * MD_RED_BLOCKS));
     HIPCHECK(hipMalloc(&g->d_tot_cnt, sizeof(unsigned long long) * 4));
     HIPCHECK(hipMalloc(&g->d_queue, sizeof(int) * (MD_NQUEUE + 1)));
     HIPCHECK(hipHostMalloc((void **)&g->h_count, sizeof(int) * s->nr_cells, hipHostMallocDefault));
@@ -2099,8 +2135,10 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
                            g->d_self_entry, g->d_inv_off, g->d_inv_list, (const float4 *)g->d_jpart, g->jcap,
                            g->d_fhi, g->d_flo);
     if (timed) HIPCHECK(hipEventRecord(g->ev[2], g->stream));
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, g->stream, g->ncells * g->nsplit, g->d_cell_acc,
-                       g->d_cell_cnt, g->d_tot, g->d_tot_cnt);
+    hipLaunchKernelGGL(k_reduce_a, dim3(MD_RED_BLOCKS), dim3(256), 0, g->stream, g->ncells * g->nsplit,
+                       g->d_cell_acc, g->d_cell_cnt, g->d_red_acc, g->d_red_cnt);
+    hipLaunchKernelGGL(k_reduce_b, dim3(1), dim3(256), 0, g->stream, MD_RED_BLOCKS, g->d_red_acc, g->d_red_cnt,
+                       g->d_tot, g->d_tot_cnt);
     if (timed) HIPCHECK(hipEventRecord(g->ev[3], g->stream));
     HIPCHECK(hipGetLastError());
     return engine_err_ok;

@@ -96,6 +96,8 @@ struct gpu_state {
     unsigned short *d_keyof;
     double *d_cell_acc, *d_tot;
     unsigned long long *d_cell_cnt, *d_tot_cnt;
+    double *d_red_acc; /* per-block partial sums of the totals */
+    unsigned long long *d_red_cnt;
     int *d_queue;
     /* pseudo-Verlet: per-cell maxdx on the device; whether the sort lists
        are those of the last sort of the current layout, and its counts */
