@@ -272,6 +272,9 @@ struct space {
     long long nr_pairs;
     /* Exclusions subtracted by the last pass (new). */
     long long nr_excl_evaluated;
+    /* Scratch of space_setcells, kept between calls (new). */
+    void *setcells_scratch;
+    size_t setcells_scratch_size;
 };
 
 int space_init(struct space *s, const double *origin, const double *dim, double *L, double cutoff,
@@ -285,7 +288,7 @@ int space_flush(struct space *s);
 /* New helper: replace the cell contents by a layout built elsewhere (the
    device step).  count[cid] particles per cell in storage order, given by
    id with their new cell-relative x, v and type; charge, vid and flags
-   follow the id.  Forces are left to the caller. */
+   follow the id.  Forces are set to zero. */
 int space_setcells(struct space *s, const int *count, const double *x, const double *v, const int *id,
                    const int *type);
 void space_free(struct space *s);

@@ -41,6 +41,7 @@
 #include <string.h>
 
 #include "gpu_state.h"
+#include "hostpack.h"
 
 /* ------------------------------------------------------------ device helpers */
 
@@ -1954,7 +1955,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
 int gpu_upload(struct engine *e) {
     struct gpu_state *g = (struct gpu_state *)e->gpu;
     struct space *s = &e->s;
-    int cid, k, n = 0, maxc = 0, maxid = -1;
+    int cid, n = 0, maxc = 0, maxid = -1;
     for (cid = 0; cid < s->nr_cells; cid++) {
         int c = s->cells[cid].count;
         if (c > maxc) maxc = c;
@@ -1964,25 +1965,9 @@ int gpu_upload(struct engine *e) {
         return gerror_(engine_err_maxparts, "more particles in a cell than the packed record holds (1024 for up to "
                                             "64 types, 256 for up to 256, 64 for up to 1024)");
     if (gpu_alloc_parts(g, n > 0 ? n : 1) < 0) return engine_err_gpu;
-    for (cid = 0; cid < s->nr_cells; cid++) {
-        struct cell *c = &s->cells[cid];
-        g->h_count[cid] = c->count;
-        g->h_offset[cid] = (cid == 0) ? 0 : g->h_offset[cid - 1] + s->cells[cid - 1].count;
-        g->h_cnt_off[cid].x = c->count;
-        g->h_cnt_off[cid].y = g->h_offset[cid];
-        for (k = 0; k < c->count; k++) {
-            int a = g->h_offset[cid] + k;
-            if (c->parts[k].type < 0 || c->parts[k].type >= g->max_type)
-                return gerror_(engine_err_range, "particle type out of range");
-            g->h_x[3 * a] = c->parts[k].x[0];
-            g->h_x[3 * a + 1] = c->parts[k].x[1];
-            g->h_x[3 * a + 2] = c->parts[k].x[2];
-            g->h_type[a] = c->parts[k].type;
-            g->h_id[a] = c->parts[k].id;
-            if (c->parts[k].id > maxid) maxid = c->parts[k].id;
-        }
-    }
-    g->h_offset[s->nr_cells] = n;
+    if (space_pack(s, g->max_type, g->h_count, g->h_offset, (int *)g->h_cnt_off, g->h_x, g->h_type, g->h_id, &maxc,
+                   &maxid) < 0)
+        return gerror_(engine_err_range, "particle type out of range");
     if (maxid + 1 > g->nids) return gerror_(engine_err_range, "particle id beyond the ids known at engine_gpu_load");
     g->nparts = n;
     g->maxcount = maxc;

@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include "gpu_state.h"
+#include "hostpack.h"
 
 struct step_geom {
     double h[3];
@@ -779,18 +780,9 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
         s->verlet_rebuild = 0;
     }
     space_prepare(s);
-    for (int cid = 0; cid < nc; cid++) {
-        struct cell *c = &s->cells[cid];
-        for (int k = 0; k < c->count; k++) {
-            const int a = ioi[c->parts[k].id];
-            if (a < 0 || a >= n) {
-                rc = gerror_(engine_err_range, "particle missing from the last pass");
-                goto done;
-            }
-            c->parts[k].f[0] += (double)g->h_fhi[a].x + (double)g->h_flo[a].x;
-            c->parts[k].f[1] += (double)g->h_fhi[a].y + (double)g->h_flo[a].y;
-            c->parts[k].f[2] += (double)g->h_fhi[a].z + (double)g->h_flo[a].z;
-        }
+    if (space_add_forces(s, ioi, n, (const float *)g->h_fhi, (const float *)g->h_flo) < 0) {
+        rc = gerror_(engine_err_range, "particle missing from the last pass");
+        goto done;
     }
     if (b->nrig > 0) {
         /* P-SHAKE's coupling matrices and the constraint forces of the last
@@ -1061,9 +1053,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
                 mass[k] = e->types[k].mass;
                 imass[k] = e->types[k].imass;
             }
-            for (int cid = 0; cid < s->nr_cells; cid++)
-                for (int k = 0; k < s->cells[cid].count; k++)
-                    for (int m = 0; m < 3; m++) v[3 * (g->h_offset[cid] + k) + m] = s->cells[cid].parts[k].v[m];
+            space_pack_v(s, g->h_offset, v);
             if (hipMemcpy(b.mass, mass, sizeof(double) * g->max_type, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(b.imass, imass, sizeof(double) * g->max_type, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(b.v, v, sizeof(double) * 3 * g->nparts, hipMemcpyHostToDevice) != hipSuccess ||

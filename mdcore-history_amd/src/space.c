@@ -362,6 +362,7 @@ int space_prepare(struct space *s) {
     s->virial = 0.0;
     s->nr_pairs = 0;
     s->nr_excl_evaluated = 0;
+#pragma omp parallel for schedule(static) private(pid, k)
     for (cid = 0; cid < s->nr_cells; cid++) {
         struct cell *c = &s->cells[cid];
         c->epot = 0.0;
@@ -415,8 +416,18 @@ int space_flush(struct space *s) {
 
 int space_setcells(struct space *s, const int *count, const double *x, const double *v, const int *id,
                    const int *type) {
-    struct part *save;
-    int cid, k, m, a, n = 0;
+    /* The fields the layout does not carry (charge, vid, flags), gathered
+       by id into a scratch array kept between calls; then every cell is
+       rewritten in slot order, forces zeroed.  Cells run in parallel: each
+       writes its own parts and its own particles' partlist/celllist
+       entries. */
+    struct meta {
+        float q;
+        int vid;
+        unsigned short flags;
+    } *meta;
+    int cid, a, n = 0, bad = 0;
+    int *start;
     if (s == NULL || count == NULL || x == NULL || v == NULL || id == NULL || type == NULL)
         return serror_(space_err_null);
     for (cid = 0; cid < s->nr_cells; cid++) {
@@ -424,36 +435,55 @@ int space_setcells(struct space *s, const int *count, const double *x, const dou
         n += count[cid];
     }
     if (n != s->nr_parts) return serror_(space_err_range);
-    if (posix_memalign((void **)&save, cell_partalign, sizeof(struct part) * (n > 0 ? n : 1)) != 0)
-        return serror_(space_err_malloc);
+    const size_t need = (sizeof(struct meta) + sizeof(int)) * (size_t)(n > 0 ? n : 1) + sizeof(int) * (s->nr_cells + 1);
+    if (s->setcells_scratch_size < need) {
+        free(s->setcells_scratch);
+        s->setcells_scratch_size = 0;
+        if ((s->setcells_scratch = malloc(need)) == NULL) return serror_(space_err_malloc);
+        s->setcells_scratch_size = need;
+    }
+    meta = (struct meta *)s->setcells_scratch;
+    start = (int *)(meta + (n > 0 ? n : 1));
+#pragma omp parallel for schedule(static) reduction(| : bad)
     for (a = 0; a < n; a++) {
         if (id[a] < 0 || id[a] >= s->size_parts || s->partlist[id[a]] == NULL) {
-            free(save);
-            return serror_(space_err_range);
+            bad = 1;
+            continue;
         }
-        save[a] = *s->partlist[id[a]];
+        const struct part *p = s->partlist[id[a]];
+        meta[a].q = p->q;
+        meta[a].vid = p->vid;
+        meta[a].flags = p->flags;
     }
+    if (bad) return serror_(space_err_range);
     for (a = 0, cid = 0; cid < s->nr_cells; cid++) {
         struct cell *c = &s->cells[cid];
-        c->count = 0;
-        if (count[cid] > c->size && cell_grow(c, count[cid]) < 0) {
-            free(save);
-            return serror_(space_err_malloc);
-        }
-        for (k = 0; k < count[cid]; k++, a++) {
+        start[cid] = a;
+        a += count[cid];
+        if (count[cid] > c->size && cell_grow(c, count[cid]) < 0) return serror_(space_err_malloc);
+    }
+#pragma omp parallel for schedule(static)
+    for (cid = 0; cid < s->nr_cells; cid++) {
+        struct cell *c = &s->cells[cid];
+        for (int k = 0; k < count[cid]; k++) {
+            const int b = start[cid] + k;
             struct part *p = &c->parts[k];
-            *p = save[a];
-            for (m = 0; m < 3; m++) {
-                p->x[m] = x[3 * a + m];
-                p->v[m] = v[3 * a + m];
+            for (int m = 0; m < 3; m++) {
+                p->x[m] = x[3 * b + m];
+                p->v[m] = v[3 * b + m];
+                p->f[m] = 0;
             }
-            p->type = (short)type[a];
+            p->x[3] = p->v[3] = p->f[3] = 0;
+            p->q = meta[b].q;
+            p->id = id[b];
+            p->vid = meta[b].vid;
+            p->type = (short)type[b];
+            p->flags = meta[b].flags;
             s->partlist[p->id] = p;
             s->celllist[p->id] = c;
         }
         c->count = count[cid];
     }
-    free(save);
     return space_err_ok;
 }
 
@@ -473,5 +503,6 @@ void space_free(struct space *s) {
     free(s->tasks);
     free(s->partlist);
     free(s->celllist);
+    free(s->setcells_scratch);
     memset(s, 0, sizeof(*s));
 }
