@@ -2205,7 +2205,7 @@ static void gpu_sorted(struct engine *e, struct gpu_state *g) {
 extern "C" int engine_nonbond_gpu(struct engine *e) {
     struct gpu_state *g;
     struct space *s;
-    int cid, k, flag = 0;
+    int flag = 0;
     if (e == NULL) return gerror_(engine_err_null, "NULL engine");
     g = (struct gpu_state *)e->gpu;
     if (g == NULL) return gerror_(engine_err_nogpu, "engine_gpu_load was not called");
@@ -2230,15 +2230,7 @@ extern "C" int engine_nonbond_gpu(struct engine *e) {
         HIPCHECK(hipEventElapsedTime(&ms, g->ev[0], g->ev[3]));
         e->timers[engine_timer_gpu_kernels] = ms * 1e-3;
     }
-    for (cid = 0; cid < s->nr_cells; cid++) {
-        struct cell *c = &s->cells[cid];
-        for (k = 0; k < c->count; k++) {
-            int a = g->h_offset[cid] + k;
-            c->parts[k].f[0] += (double)g->h_fhi[a].x + (double)g->h_flo[a].x;
-            c->parts[k].f[1] += (double)g->h_fhi[a].y + (double)g->h_flo[a].y;
-            c->parts[k].f[2] += (double)g->h_fhi[a].z + (double)g->h_flo[a].z;
-        }
-    }
+    space_add_forces_slots(s, g->h_offset, (const float *)g->h_fhi, (const float *)g->h_flo);
     s->epot_nonbond += g->h_tot[0];
     s->epot_exclusion += g->h_tot[1];
     s->epot += g->h_tot[0] + g->h_tot[1];

@@ -324,9 +324,12 @@ int engine_advance(struct engine *e) {
     int cid, pid, k, delta[3];
     double epot = 0.0, dt = e->dt, w;
     if (e->flags & engine_flag_verlet) {
+        /* no migration here: cells are independent (the energy sum stays in
+           cell order) */
+        for (cid = 0; cid < s->nr_real; cid++) epot += s->cells[s->cid_real[cid]].epot;
+#pragma omp parallel for schedule(static) private(pid, k, w)
         for (cid = 0; cid < s->nr_real; cid++) {
             struct cell *c = &s->cells[s->cid_real[cid]];
-            epot += c->epot;
             for (pid = 0; pid < c->count; pid++) {
                 struct part *p = &c->parts[pid];
                 w = dt * e->types[p->type].imass;

@@ -63,3 +63,14 @@ int space_add_forces(struct space *s, const int *ioi, int n, const float *fhi, c
     }
     return bad ? -1 : 0;
 }
+
+void space_add_forces_slots(struct space *s, const int *offset, const float *fhi, const float *flo) {
+#pragma omp parallel for schedule(static)
+    for (int cid = 0; cid < s->nr_cells; cid++) {
+        struct cell *c = &s->cells[cid];
+        for (int k = 0; k < c->count; k++) {
+            const size_t a = (size_t)(offset[cid] + k);
+            for (int m = 0; m < 3; m++) c->parts[k].f[m] += (double)fhi[4 * a + m] + (double)flo[4 * a + m];
+        }
+    }
+}

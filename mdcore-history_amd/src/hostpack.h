@@ -24,6 +24,10 @@ void space_pack_v(const struct space *s, const int *offset, double *v);
    particle.  Returns -1 if a particle has no slot in [0, n). */
 int space_add_forces(struct space *s, const int *ioi, int n, const float *fhi, const float *flo);
 
+/* p->f += fhi + flo of slot offset[c] + k for the k-th particle of cell c
+   (the layout space_pack made, unchanged since). */
+void space_add_forces_slots(struct space *s, const int *offset, const float *fhi, const float *flo);
+
 #ifdef __cplusplus
 }
 #endif
