@@ -109,7 +109,9 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
                  LDS per visit and folded per atom in a fixed order.
    Without either flag the half shell is taken when the cells hold
    engine_gpu_half_mean atoms or more on average (it saves work there; with
-   few atoms per cell its per-visit cost outweighs the halved pair work). */
+   few atoms per cell its per-visit cost outweighs the halved pair work).
+   The surrogate pass (engine_flag_mlp) takes the half shell unless
+   engine_flag_fullshell is set: its network evaluation dominates a visit. */
 #define engine_flag_fullshell 262144
 #define engine_flag_halfshell 524288
 #define engine_gpu_half_mean 48

@@ -986,6 +986,12 @@ __device__ __forceinline__ void nonbond_body(
                                 lfx -= fa * dxa;
                                 lfy -= fa * dya;
                                 lfz -= fa * dza;
+                                if constexpr (HALF) {
+                                    float *jc = jf + (lane & 3) * 3 * P.maxn;
+                                    atomicAdd(&jc[3 * ta], fa * dxa);
+                                    atomicAdd(&jc[3 * ta + 1], fa * dya);
+                                    atomicAdd(&jc[3 * ta + 2], fa * dza);
+                                }
                                 if (VK != 2 && (VK == 1 || myidx > rec_idx(wa, P.imask))) {
                                     le += ea;
                                     lv -= fa * r2a;
@@ -1008,7 +1014,8 @@ __device__ __forceinline__ void nonbond_body(
                             mlp_dense_eval(N, dn_r2, dn_p, dn_f, dn_e, T, lane);
                             wave_lds_sync();
                             for (int s = 0; s < nd; s++) {
-                                const float4 pa = stage[hitl[s * 64 + lane]];
+                                const int ta = hitl[s * 64 + lane];
+                                const float4 pa = stage[ta];
                                 const unsigned int wa = __float_as_uint(pa.w);
                                 const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
                                             dza = (xi.z - pa.z) - sf2;
@@ -1017,6 +1024,12 @@ __device__ __forceinline__ void nonbond_body(
                                 lfx -= fa * dxa;
                                 lfy -= fa * dya;
                                 lfz -= fa * dza;
+                                if constexpr (HALF) {
+                                    float *jc = jf + (lane & 3) * 3 * P.maxn;
+                                    atomicAdd(&jc[3 * ta], fa * dxa);
+                                    atomicAdd(&jc[3 * ta + 1], fa * dya);
+                                    atomicAdd(&jc[3 * ta + 2], fa * dza);
+                                }
                                 if (VK != 2 && (VK == 1 || myidx > rec_idx(wa, P.imask))) {
                                     le += ea;
                                     lv -= fa * r2a;
@@ -1278,6 +1291,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 __global__ __launch_bounds__(64) void k_nonbond_mlp(NB_PARAMS, struct mlp_net N, const int *__restrict__ ptype) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     nonbond_body<true, false>(smem, NB_ARGS, N, ptype);
+}
+
+/* Surrogate mode on the half shell: each pair's network evaluation once. */
+__global__ __launch_bounds__(64) void k_nonbond_mlp_half(NB_PARAMS, struct mlp_net N, const int *__restrict__ ptype) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    nonbond_body<true, true>(smem, NB_ARGS, N, ptype);
 }
 
 /* Half shell: each pair task once, from its i cell, with the opposite
@@ -1947,7 +1966,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
     HIPCHECK(hipHostMalloc((void **)&g->h_tot, sizeof(double) * 4, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_tot_cnt, sizeof(unsigned long long) * 4, hipHostMallocDefault));
 
-    for (int k = 0; k < 5; k++) g->lds_nb[k] = 0;
+    for (int k = 0; k < 6; k++) g->lds_nb[k] = 0;
     return engine_err_ok;
 }
 
@@ -1974,6 +1993,9 @@ int gpu_upload(struct engine *e) {
     g->half = (e->flags & engine_flag_halfshell) ? 1
               : (e->flags & engine_flag_fullshell) ? 0
                                                    : (long long)n >= (long long)engine_gpu_half_mean * s->nr_cells;
+    /* surrogate mode: the network evaluation dominates a visit, so the half
+       shell's single evaluation per pair wins at any occupancy */
+    g->mlp_half = !(e->flags & engine_flag_fullshell);
     g->P.nparts = n;
     g->P.nids = g->nids;
     g->P.maxn = maxc > 0 ? maxc : 1;
@@ -1995,7 +2017,7 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget, 3 / 4
        the half-shell table kernels */
     const int wide = g->ncells >= 4 * 16 * g->ncu;
-    const int kern = mlp ? 1 : g->half ? (wide ? 4 : 3) : (wide ? 2 : 0);
+    const int kern = mlp ? (g->mlp_half ? 5 : 1) : g->half ? (wide ? 4 : 3) : (wide ? 2 : 0);
     struct nb_params P = g->P;
     int nb;
     size_t lds = 0;
@@ -2044,6 +2066,8 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
         int per_cu = 0;
         if (kern == 1)
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_mlp, 64, lds));
+        else if (kern == 5)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_mlp_half, 64, lds));
         else if (kern == 2)
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_wide, 64, lds));
         else if (kern == 3)
@@ -2097,13 +2121,18 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
        all compute units */
     const int nitems = g->ncells * g->nsplit;
     const int grid = g->grid_nb[kern] < nitems ? g->grid_nb[kern] : nitems;
-    if (kern >= 3) {
+    if (kern == 3 || kern == 4) {
         hipLaunchKernelGGL(kern == 4 ? k_nonbond_half_wide : k_nonbond_half, dim3(grid), dim3(64), lds,
                            g->stream, P, g->d_cnt_off, g->d_nbr_h, g->d_nbr_off_h, g->d_nbs_h, g->d_origin, g->d_loc,
                            g->d_xf, g->d_x, g->d_id, g->d_sortlist, g->d_keyof, g->d_ptab,
                            (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list, g->d_index_of_id,
                            g->d_cell_of, O, g->d_queue);
-    } else if (kern == 1)
+    } else if (kern == 5)
+        hipLaunchKernelGGL(k_nonbond_mlp_half, dim3(grid), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr_h,
+                           g->d_nbr_off_h, g->d_nbs_h, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
+                           g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
+                           g->d_index_of_id, g->d_cell_of, O, g->d_queue, gpu_mlp_net(g), g->d_ptype);
+    else if (kern == 1)
         hipLaunchKernelGGL(k_nonbond_mlp, dim3(grid), dim3(64), lds, g->stream, P, g->d_cnt_off, g->d_nbr,
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,

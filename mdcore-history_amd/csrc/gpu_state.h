@@ -60,9 +60,10 @@ struct gpu_state {
     int nparts, nids;
     struct nb_params P;
     /* [0] table mode, [1] surrogate mode, [2] table mode with the 5-wave
-       budget, [3] / [4] the half-shell table kernels with 4 / 5 waves */
-    int grid_nb[5], ncu;
-    size_t lds_nb[5];
+       budget, [3] / [4] the half-shell table kernels with 4 / 5 waves, [5]
+       the half-shell surrogate kernel */
+    int grid_nb[6], ncu;
+    size_t lds_nb[6];
     /* static */
     float4 *d_ptab;
     float *d_coeffs;
@@ -75,7 +76,7 @@ struct gpu_state {
     int4 *d_nbr_h;
     float4 *d_nbs_h;
     int *d_nbr_off_h, *d_self_entry, *d_inv_off, *d_inv_list;
-    int n_entries_h, jcap, half;
+    int n_entries_h, jcap, half, mlp_half;
     float4 *d_jpart;
     /* split owners: nsplit neighbour groups per cell; per group the owner
        forces of every atom (pcap float4 allocated in each) */

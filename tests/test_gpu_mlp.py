@@ -179,3 +179,23 @@ def test_mlp_flag_requires_init():
         eng.nonbond()
     md.library().errs_clear()
     eng.close()
+
+
+def test_mlp_half_and_full_shell_agree(water_trained):
+    """The surrogate pass on the half shell (the default: each pair's network
+    evaluation once, opposite forces folded per atom) and on the full shell
+    (engine_flag_fullshell: each pair at both cells) count the same pairs and
+    agree to FP32 summation order."""
+    w = water_trained
+    eng = w["eng"]
+    out = {}
+    for label, flags in (("half", 0), ("full", md.engine_flag_fullshell)):
+        eng.setflags(md.engine_flag_mlp | flags, md.engine_flag_fullshell if flags == 0 else 0)
+        eng.nonbond()
+        out[label] = (eng.forces().copy(), eng.gpu_results())
+    eng.setflags(0, md.engine_flag_mlp | md.engine_flag_fullshell)
+    (fh, rh), (ff, rf) = out["half"], out["full"]
+    assert rh["pairs"] == rf["pairs"] and rh["exclusions"] == rf["exclusions"]
+    assert rel_rmse(fh, ff) <= 1e-5
+    assert rh["epot"] == pytest.approx(rf["epot"], rel=1e-6)
+    assert rh["virial"] == pytest.approx(rf["virial"], rel=1e-6)
