@@ -58,3 +58,28 @@ def test_full_size_parity(name):
                 assert np.array_equal(eng.sortlist(cid, sid, n), orc.sortlist(cid, sid, n)), (cid, sid)
     eng.close()
     orc.close()
+
+
+@pytest.mark.parametrize("name", sorted(FULL))
+def test_full_size_modes_agree(name):
+    """At full size, the default pass and the passes with the other pair-kernel
+    shell and without the owner split count the same pairs and exclusions and
+    agree to FP32 summation order."""
+    sysm = FULL[name]()
+    res = {}
+    for label, flags in (("default", 0), ("full", md.engine_flag_fullshell), ("half", md.engine_flag_halfshell),
+                         ("nosplit", md.engine_flag_nosplit)):
+        eng = systems.build_engine(sysm, md.engine_flag_gpu | flags)
+        eng.shuffle()
+        eng.nonbond()
+        res[label] = (eng.forces().copy(), eng.gpu_results())
+        eng.close()
+    f0, g0 = res["default"]
+    for label in ("full", "half", "nosplit"):
+        f, g = res[label]
+        err = float(np.sqrt(np.sum((f - f0) ** 2) / np.sum(f0 ** 2)))
+        print(f"{name} {label}: pairs {g['pairs']}, rel diff to default {err:.2e}", flush=True)
+        assert g["pairs"] == g0["pairs"] and g["exclusions"] == g0["exclusions"]
+        assert g["pairs_refwindow"] == g0["pairs_refwindow"]
+        assert err <= 2e-6
+        assert abs(g["epot"] - g0["epot"]) <= 1e-6 * abs(g0["epot"])
