@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 import mdcore as md
+import surrogate as sg
 import systems
 
 pytestmark = pytest.mark.gpu
@@ -95,6 +96,32 @@ def test_resident_steps_match_host_steps(name):
     moved = int(np.count_nonzero(cell0 != cell1))
     print(f"{name}: {moved} of {sysm.N} particles changed cell", flush=True)
     assert moved > 0
+    eh.close()
+    ed.close()
+
+
+def test_resident_steps_match_host_steps_surrogate():
+    """The surrogate pass (half-shell network kernel) inside resident steps:
+    a briefly trained network copied between two engines; device steps equal
+    host-driven steps bit for bit."""
+    make, dt, n = CASES["water_1k"]
+    sysm = make()
+    eh = systems.build_engine(sysm, md.engine_flag_gpu)
+    ed = systems.build_engine(sysm, md.engine_flag_gpu)
+    dom = sg.pair_domains(eh, sysm)
+    S = sg.make_samples(eh, sysm, 1 << 16, dom[0], dom[1], seed=3)
+    eh.mlp_init(*dom, seed=4)
+    eh.mlp_train(*S, steps=300, batch_tiles=64, seed=5)
+    ed.mlp_init(*dom, seed=4)
+    ed.mlp_params(eh.mlp_params())
+    for e in (eh, ed):
+        e.setdt(dt)
+        e.setflags(md.engine_flag_mlp, 0)
+    epot_h, _ = host_steps(eh, sysm, 10)
+    epot_d, _ = ed.gpu_steps(10)
+    assert np.all(np.isfinite(epot_h))
+    assert np.array_equal(epot_h, epot_d)
+    assert_same_state(eh, ed, sysm)
     eh.close()
     ed.close()
 
