@@ -1447,6 +1447,7 @@ static void gpu_free_pass(struct gpu_state *g) {
     (void)hipHostFree(g->h_id);
     (void)hipHostFree(g->h_fhi);
     (void)hipHostFree(g->h_flo);
+    (void)hipHostFree(g->h_v);
     g->d_count = g->d_offset = NULL;
     g->d_x = NULL;
     g->d_type = g->d_id = g->d_cell_of = NULL;
@@ -1456,6 +1457,7 @@ static void gpu_free_pass(struct gpu_state *g) {
     g->h_x = NULL;
     g->h_type = g->h_id = NULL;
     g->h_fhi = g->h_flo = NULL;
+    g->h_v = NULL;
     g->nparts_cap = 0;
 }
 
@@ -1481,6 +1483,7 @@ static int gpu_alloc_parts(struct gpu_state *g, int n) {
     HIPCHECK(hipHostMalloc((void **)&g->h_id, sizeof(int) * N, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_fhi, sizeof(float4) * N, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_flo, sizeof(float4) * N, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void **)&g->h_v, sizeof(double) * 3 * N, hipHostMallocDefault));
     g->nparts_cap = (int)N;
     return 0;
 }
@@ -1488,7 +1491,10 @@ static int gpu_alloc_parts(struct gpu_state *g, int n) {
 static int gpu_alloc_ids(struct gpu_state *g, int nids) {
     if (nids <= g->nids_cap) return 0;
     (void)hipFree(g->d_index_of_id);
+    (void)hipHostFree(g->h_ioi);
+    g->h_ioi = NULL;
     HIPCHECK(hipMalloc(&g->d_index_of_id, sizeof(int) * (size_t)nids));
+    HIPCHECK(hipHostMalloc((void **)&g->h_ioi, sizeof(int) * (size_t)nids, hipHostMallocDefault));
     g->nids_cap = nids;
     return 0;
 }
@@ -1536,6 +1542,7 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_excl_off);
     (void)hipFree(g->d_excl_list);
     (void)hipFree(g->d_index_of_id);
+    (void)hipHostFree(g->h_ioi);
     (void)hipFree(g->d_cnt_off);
     (void)hipFree(g->d_cell_acc);
     (void)hipFree(g->d_cell_cnt);

@@ -110,6 +110,8 @@ struct gpu_state {
     int *h_type, *h_id, *h_count, *h_offset;
     int2 *h_cnt_off;
     float4 *h_fhi, *h_flo;
+    double *h_v; /* pinned: velocities to and from the resident steps */
+    int *h_ioi;  /* pinned: id -> slot of the last pass, downloaded */
     double *h_tot;
     unsigned long long *h_tot_cnt;
     /* species pairs = distinct potentials, in table order */

@@ -721,13 +721,15 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
                          double *ekin, int verlet, double smaxdx) {
     struct space *s = &e->s;
     const int n = g->nparts, nc = g->ncells;
-    const size_t M = (size_t)(n > 0 ? n : 1);
     int rc = engine_err_ok;
-    double *x = (double *)malloc(sizeof(double) * 3 * M), *v = (double *)malloc(sizeof(double) * 3 * M);
+    /* pinned staging (allocated with the particle buffers) for the bulk
+       downloads; the cell counts get their own array, since h_count and
+       h_offset describe the layout of the last upload */
+    double *x = g->h_x, *v = g->h_v;
     double *es = (double *)malloc(sizeof(double) * 4 * (size_t)nsteps);
-    int *id = (int *)malloc(sizeof(int) * M), *type = (int *)malloc(sizeof(int) * M);
-    int *count = (int *)malloc(sizeof(int) * (size_t)nc), *ioi = (int *)malloc(sizeof(int) * (size_t)g->nids);
-    double *ox = verlet ? (double *)malloc(sizeof(double) * 3 * M) : NULL;
+    int *id = g->h_id, *type = g->h_type, *ioi = g->h_ioi;
+    int *count = (int *)malloc(sizeof(int) * (size_t)nc);
+    double *ox = verlet ? (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1)) : NULL;
     double *cm = verlet ? (double *)malloc(sizeof(double) * (size_t)nc) : NULL;
     if (x == NULL || v == NULL || es == NULL || id == NULL || type == NULL || count == NULL || ioi == NULL ||
         (verlet && (ox == NULL || cm == NULL))) {
@@ -827,13 +829,8 @@ static int step_download(struct engine *e, struct gpu_state *g, struct step_bufs
 done:
     free(ox);
     free(cm);
-    free(x);
-    free(v);
     free(es);
-    free(id);
-    free(type);
     free(count);
-    free(ioi);
     return rc;
 }
 
@@ -1045,8 +1042,8 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     {
         double *mass = (double *)malloc(sizeof(double) * g->max_type);
         double *imass = (double *)malloc(sizeof(double) * g->max_type);
-        double *v = (double *)malloc(sizeof(double) * 3 * (size_t)(g->nparts > 0 ? g->nparts : 1));
-        if (mass == NULL || imass == NULL || v == NULL) {
+        double *v = g->h_v; /* pinned */
+        if (mass == NULL || imass == NULL) {
             rc = gerror_(engine_err_malloc, "malloc failed");
         } else {
             for (int k = 0; k < g->max_type; k++) {
@@ -1056,13 +1053,12 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
             space_pack_v(s, g->h_offset, v);
             if (hipMemcpy(b.mass, mass, sizeof(double) * g->max_type, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(b.imass, imass, sizeof(double) * g->max_type, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(b.v, v, sizeof(double) * 3 * g->nparts, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpyAsync(b.v, v, sizeof(double) * 3 * g->nparts, hipMemcpyHostToDevice, g->stream) != hipSuccess ||
                 hipMemsetAsync(g->d_queue + MD_NQUEUE, 0, sizeof(int), g->stream) != hipSuccess)
                 rc = gerror_(engine_err_gpu, "upload failed");
         }
         free(mass);
         free(imass);
-        free(v);
     }
     const int verlet = (e->flags & engine_flag_verlet) != 0;
     double smaxdx = s->maxdx;
