@@ -690,7 +690,8 @@ __device__ __forceinline__ void nonbond_body(
         }
         /* Wave-uniform from here: the neighbour list is read with scalar loads. */
         const int item = __builtin_amdgcn_readfirstlane(lo_q + it);
-        const int c = item / P.nsplit, grp = item - c * P.nsplit;
+        const int ci = item / P.nsplit, grp = item - ci * P.nsplit;
+        const int c = P.order[ci];
         const int2 co_c = cnt_off[c];
         const int nc = co_c.x, offc = co_c.y;
         if (nc == 0) {
@@ -1551,6 +1552,7 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_red_cnt);
     (void)hipFree(g->d_tot_cnt);
     (void)hipFree(g->d_queue);
+    (void)hipFree(g->d_order);
     (void)hipHostFree(g->h_count);
     (void)hipFree(g->d_cell_maxdx);
     (void)hipHostFree(g->h_cell_maxdx);
@@ -1954,6 +1956,31 @@ extern "C" int engine_gpu_load(struct engine *e) {
         if (g->nsplit > MD_SPLIT_MAX) g->nsplit = MD_SPLIT_MAX;
     }
     g->P.nsplit = g->nsplit;
+    {
+        /* Owner cells in blocks of 6x6x6, blocks and the cells inside them in
+           raster order: the owners a compute die works on at once, and their
+           neighbours, stay close together in its L2. */
+        int *ord = (int *)malloc(sizeof(int) * (size_t)s->nr_cells);
+        if (ord == NULL) return gerror_(engine_err_malloc, "malloc failed");
+        const int B = 6, nbx = (s->cdim[0] + B - 1) / B, nby = (s->cdim[1] + B - 1) / B,
+                  nbz = (s->cdim[2] + B - 1) / B;
+        int q = 0;
+        for (int bx = 0; bx < nbx; bx++)
+            for (int by = 0; by < nby; by++)
+                for (int bz = 0; bz < nbz; bz++)
+                    for (int x = bx * B; x < (bx + 1) * B && x < s->cdim[0]; x++)
+                        for (int y = by * B; y < (by + 1) * B && y < s->cdim[1]; y++)
+                            for (int z = bz * B; z < (bz + 1) * B && z < s->cdim[2]; z++)
+                                ord[q++] = space_cellid(s, x, y, z);
+        if (q != s->nr_cells) {
+            free(ord);
+            return gerror_(engine_err_range, "cell order");
+        }
+        HIPCHECK(hipMalloc(&g->d_order, sizeof(int) * (size_t)s->nr_cells));
+        HIPCHECK(hipMemcpy(g->d_order, ord, sizeof(int) * (size_t)s->nr_cells, hipMemcpyHostToDevice));
+        free(ord);
+        g->P.order = g->d_order;
+    }
     HIPCHECK(hipMalloc(&g->d_cnt_off, sizeof(int2) * s->nr_cells));
     HIPCHECK(hipMalloc(&g->d_cell_acc, sizeof(double) * 3 * (size_t)s->nr_cells * g->nsplit));
     HIPCHECK(hipMalloc(&g->d_cell_cnt, sizeof(unsigned long long) * 3 * (size_t)s->nr_cells * g->nsplit));

@@ -43,6 +43,7 @@ struct nb_params {
     /* dynamic LDS layout (byte offsets) */
     int lds_own_key, lds_kfl, lds_stage, lds_hit, lds_dense, lds_jf;
     int nsplit; /* neighbour groups per owner cell (work items per cell) */
+    const int *order; /* owner cell of each owner position in the work order */
 };
 
 /* Neighbour-list entry of an owner cell: x = other cell, y = sid | orient<<4
@@ -100,6 +101,7 @@ struct gpu_state {
     double *d_red_acc; /* per-block partial sums of the totals */
     unsigned long long *d_red_cnt;
     int *d_queue;
+    int *d_order; /* owner cells in blocked work order */
     /* pseudo-Verlet: per-cell maxdx on the device; whether the sort lists
        are those of the last sort of the current layout, and its counts */
     double *d_cell_maxdx, *h_cell_maxdx;
