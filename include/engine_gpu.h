@@ -102,7 +102,8 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
    r_lo[p] <= r <= cutoff through the network and the others through the
    tables. */
 #define engine_flag_mlp 131072
-/* Pair-kernel shell, chosen at engine_gpu_load (no reference counterpart):
+/* Pair-kernel shell, chosen at every particle upload (each pass, each
+   engine_gpu_steps call; no reference counterpart):
    full shell    every pair evaluated by both of its atoms' cells, forces
                  kept in registers;
    half shell    every pair task visited once, the opposite forces added in
