@@ -1709,7 +1709,10 @@ extern "C" int engine_gpu_load(struct engine *e) {
         P->coef_bytes = (int)(sizeof(float) * 8 * (nchunks > 0 ? nchunks : 1));
         P->ncells = s->nr_cells;
         P->max_type = mt;
-        P->proj = (e->flags & engine_flag_projwindow) ? 1 : 0;
+        /* engine_flag_unsorted: the reference's unsorted runner (runner.c:801-804,
+           runner_dopair.c:1049-1290) evaluates every pair within the cutoff,
+           which is the pair set of the projected window */
+        P->proj = (e->flags & (engine_flag_projwindow | engine_flag_unsorted)) ? 1 : 0;
         P->bias = sqrt(s->h[0] * s->h[0] + s->h[1] * s->h[1] + s->h[2] * s->h[2]);
         P->dscale = (double)32767 / (2 * P->bias);
         P->dmaxdist = (int)(2 + P->dscale * (s->cutoff + 0.0 + 0.0));

@@ -188,3 +188,22 @@ def test_shell_choice(name, expect):
     other = "half" if expect == "full" else "full"
     assert np.array_equal(out["auto"], out[expect])
     assert not np.array_equal(out["auto"], out[other])
+
+
+def test_unsorted_flag_evaluates_every_pair():
+    """engine_flag_unsorted (the reference's unsorted runner, runner.c:801-804)
+    evaluates every pair within the cutoff: on the span-2 mixture, where the
+    reference's sorted window misses some, the pass equals the brute force."""
+    sys = SYSTEMS["mixture_4k"]()
+    orc = po.from_system(sys)
+    eng = engine_pass(sys, md.engine_flag_gpu | md.engine_flag_unsorted)
+    try:
+        g = eng.gpu_results()
+        b = orc.bruteforce()
+        r = orc.nonbond(po.OR_WINDOW_REF)
+        assert g["pairs"] == b["pairs"] > r["pairs"]
+        assert rel_rmse(eng.forces(), b["f"]) <= FORCE_RTOL
+        assert abs(g["epot"] - b["epot"]) <= ENERGY_RTOL * abs(b["epot"])
+    finally:
+        eng.close()
+        orc.close()
