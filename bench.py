@@ -267,7 +267,7 @@ def mlp_section(sysm, eng, args, f_table, f64, table_value):
                      "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
                      "sample_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": None, "kernel": "k_nonbond_mlp",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": None, "kernel": "k_nonbond_mlp_half + k_fold",
                      "kernel_ms": round(ms["nonbond_ms"], 4),
                      "note": "bf16 MFMA FLOP of the 64x64 layer per network evaluation; VALU work (SiLU, "
                              "layer 1, heads, pair search) not counted"},
