@@ -50,6 +50,7 @@ MI355X_CUS = 256            # the pair kernel's 5-wave build runs from 64 cells 
 DT_PS = 0.002               # 2 fs time step for ns/day
 METRIC = "pair-interactions/s & ns/day at 1M atoms; MLP vs table at fixed force RMSE"
 TRAFFIC_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_traffic.json"
+COUNTERS_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_counters.json"
 
 
 def build_workload(name):
@@ -156,11 +157,23 @@ def main():
                 traffic = t.get("bytes_per_launch")
         except Exception:
             traffic = None
+    valu = None
+    if COUNTERS_FILE.exists():
+        try:
+            c = json.loads(COUNTERS_FILE.read_text())
+            if c.get("workload") == args.workload and c.get("window") == args.window:
+                valu = c.get("valu_issue_frac")
+        except Exception:
+            valu = None
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "kernel": pair_kernel_name(sysm, md),
                 "kernel_ms": round(ms["nonbond_ms"], 4),
-                "note": "FP32 vector pipe (VALU); packed FP32 disabled, so 78.65 TFLOP/s is the issue ceiling"}
+                "valu_issue_frac": valu,
+                "note": "FP32 vector pipe (VALU); packed FP32 disabled, so 78.65 TFLOP/s is the issue ceiling; "
+                        "valu_issue_frac = VALU instructions / (CUs x cycles) from the committed counter "
+                        "summary: the kernel is VALU-issue-bound, with most instructions outside the 46 "
+                        "FLOP of a pair evaluation"}
 
     out = {
         "metric": METRIC,
