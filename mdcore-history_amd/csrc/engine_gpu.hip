@@ -1,7 +1,7 @@
 /*
  * gfx950 nonbonded engine behind the mdcore C API (C-ABI in engine_gpu.h).
  *
- * One nonbonded pass is four launches on one stream:
+ * One nonbonded pass is a chain of launches on one stream:
  *   k_pack      cell-ordered FP64 positions -> float4 (x, y, z, type bits),
  *               id -> slot map, slot -> cell map;
  *   k_sort_rank runner_dosort (runner_dosort.c:97-161) for every cell and
@@ -11,7 +11,9 @@
  *               the reference's descending quicksort (runner.c:110-174)
  *               replayed, so that the sorted lists, ties included, are the
  *               reference's (k_sort: the same per list in global memory, for
- *               cells of more than 191 particles);
+ *               cells of more than 191 particles).  Cells of more than 48
+ *               particles get a wave per direction and a wave-parallel
+ *               restatement of that quicksort (sort_desc_wave);
  *   k_nonbond   persistent kernel: each wave pulls an owner cell from one of
  *               8 device queues (one per XCD slab of cells), walks the cell's
  *               neighbour list (the reference's self/pair tasks that touch
@@ -28,11 +30,19 @@
  *               (potential_eval.h:67-104), forces summed in registers and
  *               folded into two-sum (hi, lo) accumulators per neighbour.
  *               Excluded pairs are subtracted afterwards as in
- *               exclusion.c:81-282;
+ *               exclusion.c:81-282.  The queues hand out work items: owner
+ *               cells in 6x6x6 blocks, each split over up to 8 neighbour
+ *               groups when there are few, densely filled cells;
+ *   k_nonbond_half (and the surrogate k_nonbond_mlp_half): each pair task
+ *               is visited once, at its i cell; the opposite forces go into
+ *               LDS copies of the visited cell and out to a per-visit slot;
+ *   k_fold      per atom, the groups' partial forces and the half-shell
+ *               slots, added in a fixed order;
  *   k_reduce_a/b per-item FP64 energies/virials/counts -> totals, fixed order.
- * Every pair is evaluated by both of its cells (once per owner) with
- * bitwise-opposite dx, so the two owners see exactly opposite forces;
- * energy, virial and counts are taken on the task's i side only.
+ * In the full-shell kernels every pair is evaluated by both of its cells
+ * (once per owner) with bitwise-opposite dx, so the two owners see exactly
+ * opposite forces; energy, virial and counts are taken on the task's i side
+ * only.
  */
 #include <hip/hip_runtime.h>
 
@@ -2052,7 +2062,8 @@ int gpu_upload(struct engine *e) {
    mode of the pair kernel. */
 int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget, 3 / 4
-       the half-shell table kernels */
+       the half-shell table kernels (4 / 5 waves), 5 the half-shell
+       surrogate kernel */
     const int wide = g->ncells >= 4 * 16 * g->ncu;
     const int kern = mlp ? (g->mlp_half ? 5 : 1) : g->half ? (wide ? 4 : 3) : (wide ? 2 : 0);
     struct nb_params P = g->P;
