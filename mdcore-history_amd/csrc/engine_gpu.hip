@@ -361,8 +361,9 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
 /* The quicksort of sort_desc_t, run by a whole wave on one LDS list with the
    same result, element for element.
    - Segments below 16 elements (selection sort): each step takes the first
-     maximal key of the rest, a wave-wide max of key << 6 | (63 - index), and
-     swaps it to the front.
+     maximal key of the rest, the max of key << 6 | (63 - index) over the
+     leaf's lanes (a DPP reduction within the first row of 16 lanes), and
+     swaps it to the front with lane reads.
    - Partition (pivot p = middle key): the serial scan's k-th swap pairs the
      k-th element from the left with key <= p (ltab[k]) and the k-th from the
      right with key >= p (rtab[k]), for k = 1..K while ltab[k] <= rtab[k];
@@ -372,12 +373,21 @@ __global__ __launch_bounds__(256) void k_sort(struct nb_params P, const int *__r
      elements stop the pointers), without a further swap.
    The stack (same pushes and overflow test as sort_desc_t) lives in LDS and
    is written by lane 0.  ltab / rtab: maxn + 1 entries each. */
-__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned int u = (unsigned int)__shfl_xor((int)v, o);
-        v = u > v ? u : v;
-    }
-    return v;
+
+/* Largest value over lanes 0..15, wave-uniform: four DPP row shifts (lane l
+   takes the max of lanes l-1, l-2..l-3, l-4..l-7, l-8..l-15 of its row;
+   lanes shifted in from outside the row read 0), then lane 15. */
+__device__ __forceinline__ unsigned int row0_max_u32(unsigned int v) {
+    unsigned int u;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true); /* row_shr:1 */
+    v = u > v ? u : v;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true); /* row_shr:2 */
+    v = u > v ? u : v;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true); /* row_shr:4 */
+    v = u > v ? u : v;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true); /* row_shr:8 */
+    v = u > v ? u : v;
+    return (unsigned int)__builtin_amdgcn_readlane((int)v, 15);
 }
 
 __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, unsigned short *ltab,
@@ -402,11 +412,13 @@ __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, u
         if (hi - lo < 15) {
             const int len = hi - lo + 1;
             unsigned int w = lane < len ? a[lo + lane] : 0u;
+            /* the leaf lies in lanes 0..14: a row-0 max and lane reads */
             for (int i = 0; i < len - 1; i++) {
                 const unsigned int pk =
                     (lane >= i && lane < len) ? (((w & 0xffffu) << 6) | (unsigned int)(63 - lane)) : 0u;
-                const int im = 63 - (int)(wave_max_u32(pk) & 63u);
-                const unsigned int wi = (unsigned int)__shfl((int)w, i), wm = (unsigned int)__shfl((int)w, im);
+                const int im = 63 - (int)(row0_max_u32(pk) & 63u);
+                const unsigned int wi = (unsigned int)__builtin_amdgcn_readlane((int)w, i);
+                const unsigned int wm = (unsigned int)__builtin_amdgcn_readlane((int)w, im);
                 if (lane == i)
                     w = wm;
                 else if (lane == im)
