@@ -512,11 +512,12 @@ __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, u
    included, so those particles are placed by counting.  Only the order of
    equal keys depends on the quicksort's partitions: a list with equal keys
    is sorted again with the reference's algorithm (runner.c:110-174), in LDS,
-   and written whole: by a whole wave (sort_desc_wave) when the cell has a
-   wave per direction, else by one lane.  For cells of at
-   most MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions, 13 * maxn list
+   and written whole: by a whole wave (sort_desc_wave), or by one lane when
+   a one-wave cell has more than MD_SORT_COOP_LISTS such lists.  For cells of
+   at most MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions, 13 * maxn list
    words, 13 quicksort stacks, 13 pairs of partition tables. */
 #define MD_SORT_LDS_MAX 191
+#define MD_SORT_COOP_LISTS 7
 __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
                                                   const int *__restrict__ offset,
                                                   const unsigned int *__restrict__ sortflags,
@@ -564,8 +565,10 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
     }
     __syncthreads();
     /* Lists with a repeated key: the reference's quicksort.  With 13 waves
-       per cell, wave l sorts list l cooperatively (sort_desc_wave); with one
-       wave per cell (small cells, few such lists), lane l sorts list l. */
+       per cell, wave l sorts list l cooperatively (sort_desc_wave).  With one
+       wave per cell (small cells), the wave sorts up to MD_SORT_COOP_LISTS
+       such lists one after another; above that, lane l sorts list l, the
+       13 serial sorts overlapping their LDS latencies. */
     const int wave = lane >> 6, nwave = (int)blockDim.x >> 6, wl = lane & 63;
     if (nwave > 1) {
         for (int l = wave; l < 13; l += nwave) {
@@ -576,11 +579,25 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
             sort_desc_wave(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
             for (int i = wl; i < n; i += 64) out[l * n + i] = L[i];
         }
-    } else if (lane < 13 && ((fl >> lane) & 1u) && tie[lane]) {
-        unsigned int *L = lists + lane * n;
-        int *st = stk + lane * 2 * MD_SORT_STACK;
-        sort_desc_t(L, n, st, st + MD_SORT_STACK, flag);
-        for (int i = 0; i < n; i++) out[lane * n + i] = L[i];
+    } else {
+        unsigned int tm = 0;
+        for (int l = 0; l < 13; l++) tm |= (unsigned int)(tie[l] != 0) << l;
+        tm &= fl;
+        if (__builtin_popcount(tm) <= MD_SORT_COOP_LISTS) {
+            for (; tm; tm &= tm - 1) {
+                const int l = __builtin_ctz(tm);
+                unsigned int *L = lists + l * n;
+                int *st = stk + l * 2 * MD_SORT_STACK;
+                unsigned short *tb = tabs + (size_t)l * 2 * (maxn + 1);
+                sort_desc_wave(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
+                for (int i = wl; i < n; i += 64) out[l * n + i] = L[i];
+            }
+        } else if (lane < 13 && ((tm >> lane) & 1u)) {
+            unsigned int *L = lists + lane * n;
+            int *st = stk + lane * 2 * MD_SORT_STACK;
+            sort_desc_t(L, n, st, st + MD_SORT_STACK, flag);
+            for (int i = 0; i < n; i++) out[lane * n + i] = L[i];
+        }
     }
 }
 
