@@ -89,7 +89,7 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
 /* ------------------------------------------------------------------------
    MLP surrogate of the pair potential.  No reference counterpart: the
    reference evaluates only its piecewise-polynomial tables
-   (potential_eval.h:67-104), which remain the judge and the fallback.
+   (potential_eval.h:67-104), which remain the measure of its error and the fallback.
 
    A species pair is a distinct potential of the engine; pair ids number
    them in the order engine_gpu_load found them.  Per pair p the network

@@ -1,7 +1,7 @@
 /*
  * MLP surrogate of the interpolated pair potential (no reference
  * counterpart; the reference evaluates only the tables of
- * potential_eval.h:67-104, which stay the judge and the fallback here).
+ * potential_eval.h:67-104, which stay the measure of its error and the fallback here).
  *
  * Training runs on the GPU with this file's own kernels:
  *   k_mlp_grad     forward + backward of tiles of 16 samples of one species

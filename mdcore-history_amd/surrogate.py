@@ -2,7 +2,7 @@
 
 The surrogate replaces the table evaluation of potential_eval.h:67-104 for
 pairs inside each species pair's domain [r_lo, rc]; the tables stay the
-judge (targets, force error) and the fallback (pairs below r_lo).
+measure (targets, force error) and the fallback (pairs below r_lo).
 
   pair_domains   per species pair: r_lo where |E| first exceeds a cap
                  (the repulsive core stays on the table), r_hi = rc, and the

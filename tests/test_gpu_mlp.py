@@ -5,7 +5,7 @@
   rounding points (oracle/mlp_oracle.py, itself pinned by finite
   differences on the CPU);
 - training on the GPU: what the trained network gains over the same
-  network untrained, judged by the FP64 tables;
+  network untrained, measured against the FP64 tables;
 - the surrogate pair pass (engine_flag_mlp): same pairs and exclusions as
   the table pass, excluded pairs cancel exactly, Newton's third law,
   bitwise repeatable, force error against the table pass bounded.

@@ -245,7 +245,7 @@ def test_sort_reuse_window_widening():
     re-sort, the projected window still finds every in-cutoff pair when
     widened by each cell's largest displacement, and misses some without the
     widening; a fresh sort needs none.  Brute force on the moved positions is
-    the judge."""
+    the check."""
     sysm = systems.argon(6, jitter=0.05)  # 864 atoms, 3^3 cells of 1.13 nm
     o = po.from_system(sysm)
     o.nonbond(po.OR_WINDOW_PROJ)  # sorts at the start positions
