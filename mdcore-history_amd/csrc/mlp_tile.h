@@ -87,11 +87,14 @@ __device__ __forceinline__ void mlp_inputs(const float4 sc, float r, float &rn, 
    of that pair in every lane.  With KEEP, the lane's z1/z2/h2 and the bf16
    h1 operands are kept for the backward pass; without it (inference) each
    16-unit block goes layer 2 -> SiLU -> heads as soon as its MFMA is done,
-   so few registers stay live.  Both variants do the same arithmetic in the
-   same order, so they give the same bits. */
-template <bool KEEP>
-__device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, float r, int p, int lane, float *z1,
-                                                mlp_bf16x8 *h1b, float *z2, float *h2, float &y0, float &y1) {
+   so few registers stay live.  With REG, layer 2's A operands and biases
+   come from registers the caller loaded once (mlp_load_layer2) instead of
+   memory.  All variants do the same arithmetic in the same order, so they
+   give the same bits. */
+template <bool KEEP, bool REG>
+__device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, const mlp_bf16x8 *w2r, const float4 *b2r, float r,
+                                                int p, int lane, float *z1, mlp_bf16x8 *h1b, float *z2, float *h2,
+                                                float &y0, float &y1) {
     const float4 sc = N.sc[p];
     float rn, irn;
     mlp_inputs(sc, r, rn, irn);
@@ -114,10 +117,12 @@ __device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, float r, int p
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2f[(mb * 2 + 0) * 64 + lane], hb[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2f[(mb * 2 + 1) * 64 + lane], hb[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(REG ? w2r[mb * 2 + 0] : N.w2f[(mb * 2 + 0) * 64 + lane], hb[0],
+                                                      acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(REG ? w2r[mb * 2 + 1] : N.w2f[(mb * 2 + 1) * 64 + lane], hb[1],
+                                                      acc, 0, 0, 0);
         const int u0 = 16 * mb + 4 * (lane >> 4);
-        const float4 bb = *(const float4 *)(N.b2 + u0);
+        const float4 bb = REG ? b2r[mb] : *(const float4 *)(N.b2 + u0);
         float hh[4];
         hh[0] = mlp_silu(acc[0] + bb.x);
         hh[1] = mlp_silu(acc[1] + bb.y);
@@ -147,12 +152,28 @@ __device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, float r, int p
 /* Training forward: keeps the activations of this lane's 16 units. */
 __device__ __forceinline__ void mlp_forward16(const mlp_net &N, float r, int p, int lane, float z1[16],
                                               mlp_bf16x8 h1b[2], float z2[16], float h2[16], float &y0, float &y1) {
-    mlp_forward16_t<true>(N, r, p, lane, z1, h1b, z2, h2, y0, y1);
+    mlp_forward16_t<true, false>(N, nullptr, nullptr, r, p, lane, z1, h1b, z2, h2, y0, y1);
 }
 
 /* Inference forward. */
 __device__ __forceinline__ void mlp_infer16(const mlp_net &N, float r, int p, int lane, float &y0, float &y1) {
-    mlp_forward16_t<false>(N, r, p, lane, nullptr, nullptr, nullptr, nullptr, y0, y1);
+    mlp_forward16_t<false, false>(N, nullptr, nullptr, r, p, lane, nullptr, nullptr, nullptr, nullptr, y0, y1);
+}
+
+/* Layer 2's operands of this lane, for a run of tiles: w2r[mb * 2 + kb]
+   = A operand (mb, kb), b2r[mb] = biases of the lane's 4 rows of block mb
+   (32 + 16 VGPRs; no reload per tile). */
+__device__ __forceinline__ void mlp_load_layer2(const mlp_net &N, int lane, mlp_bf16x8 w2r[8], float4 b2r[4]) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) w2r[i] = N.w2f[i * 64 + lane];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) b2r[mb] = *(const float4 *)(N.b2 + 16 * mb + 4 * (lane >> 4));
+}
+
+/* Inference forward with layer 2's operands from mlp_load_layer2. */
+__device__ __forceinline__ void mlp_infer16_reg(const mlp_net &N, const mlp_bf16x8 w2r[8], const float4 b2r[4], float r,
+                                                int p, int lane, float &y0, float &y1) {
+    mlp_forward16_t<false, true>(N, w2r, b2r, r, p, lane, nullptr, nullptr, nullptr, nullptr, y0, y1);
 }
 
 __device__ __forceinline__ float mlp_sinh(float y) {
