@@ -203,14 +203,25 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
     if (lane == 0) atomicAdd(loss, lsum * inv_n);
 }
 
-/* Sum the per-wave partial rows of k_mlp_grad into the gradient, wave by
-   wave in order. */
-__global__ __launch_bounds__(256) void k_mlp_wsum(const float *__restrict__ partial, int nwaves,
-                                                  float *__restrict__ grad) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= MLP_WPART) return;
+/* Sum the per-wave partial rows of k_mlp_grad into the gradient.  Block b
+   takes columns [64 b, 64 b + 64); its wave s sums rows s, s + MLP_WSUM_SPLIT,
+   ... in order, and the MLP_WSUM_SPLIT sums are then added in wave order, so
+   every run gives the same bits.  (One thread per column summing all rows
+   left 17 blocks each waiting on ~500 dependent loads: 113 us per step.) */
+#define MLP_WSUM_SPLIT 16
+__global__ __launch_bounds__(64 * MLP_WSUM_SPLIT) void k_mlp_wsum(const float *__restrict__ partial, int nwaves,
+                                                                  float *__restrict__ grad) {
+    __shared__ float part[MLP_WSUM_SPLIT][64];
+    const int col = threadIdx.x & 63, s = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + col;
     float acc = 0.f;
-    for (int w = 0; w < nwaves; w++) acc += partial[(size_t)w * MLP_WPART + t];
+    if (t < MLP_WPART)
+        for (int w = s; w < nwaves; w += MLP_WSUM_SPLIT) acc += partial[(size_t)w * MLP_WPART + t];
+    part[s][col] = acc;
+    __syncthreads();
+    if (s != 0 || t >= MLP_WPART) return;
+    acc = 0.f;
+    for (int k = 0; k < MLP_WSUM_SPLIT; k++) acc += part[k][col];
     if (t < MLP_H * MLP_H)
         grad[MLP_OFF_W2 + t] += acc;
     else if (t < MLP_H * MLP_H + 2 * MLP_H)
@@ -475,8 +486,8 @@ static int mlp_gradient(struct gpu_state *g, const int *tiles, int ntiles, float
     hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64), 0, g->stream, gpu_mlp_net(g), g->d_sr, g->d_sp, g->d_sE,
                        g->d_sF, tiles, ntiles, wE, wF, 1.0f / (16.0f * ntiles), np, g->d_grad, g->d_gb1e, loss,
                        g->d_wpart);
-    hipLaunchKernelGGL(k_mlp_wsum, dim3((MLP_WPART + 255) / 256), dim3(256), 0, g->stream, g->d_wpart, grid,
-                       g->d_grad);
+    hipLaunchKernelGGL(k_mlp_wsum, dim3((MLP_WPART + 63) / 64), dim3(64 * MLP_WSUM_SPLIT), 0, g->stream, g->d_wpart,
+                       grid, g->d_grad);
     hipLaunchKernelGGL(k_mlp_chain, dim3((MLP_H + MLP_H * MLP_EMB + np * MLP_EMB + 255) / 256), dim3(256), 0,
                        g->stream, g->d_theta, np, g->d_gb1e, g->d_grad);
     HIPCHECK(hipGetLastError());
