@@ -230,26 +230,39 @@ __global__ __launch_bounds__(64 * MLP_WSUM_SPLIT) void k_mlp_wsum(const float *_
         grad[MLP_OFF_B2 + t - MLP_H * MLP_H - 2 * MLP_H] += acc;
 }
 
-/* b1e[p][u] = b1[u] + sum_e E1[u][e] emb[p][e]: gradients of b1, E1, emb.
-   One thread per output gradient, each a fixed-order sum. */
-__global__ __launch_bounds__(256) void k_mlp_chain(const float *__restrict__ theta, int np,
-                                                   const float *__restrict__ gb1e, float *__restrict__ grad) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < MLP_H) {
-        float b = 0.f;
-        for (int p = 0; p < np; p++) b += gb1e[p * MLP_H + t];
-        grad[MLP_OFF_B1 + t] += b;
-    } else if (t < MLP_H + MLP_H * MLP_EMB) {
-        const int q = t - MLP_H, u = q / MLP_EMB, e = q % MLP_EMB;
-        float acc = 0.f;
-        for (int p = 0; p < np; p++) acc += gb1e[p * MLP_H + u] * theta[MLP_OFF_EMB + p * MLP_EMB + e];
-        grad[MLP_OFF_E1 + q] += acc;
-    } else if (t < MLP_H + MLP_H * MLP_EMB + np * MLP_EMB) {
-        const int q = t - MLP_H - MLP_H * MLP_EMB, p = q / MLP_EMB, e = q % MLP_EMB;
-        float acc = 0.f;
-        for (int u = 0; u < MLP_H; u++) acc += gb1e[p * MLP_H + u] * theta[MLP_OFF_E1 + u * MLP_EMB + e];
-        grad[MLP_OFF_EMB + q] += acc;
+/* b1e[p][u] = b1[u] + sum_e E1[u][e] emb[p][e]: gradients of b1, E1, emb,
+   each a fixed-order sum. */
+__global__ __launch_bounds__(64 * MLP_WSUM_SPLIT) void k_mlp_chain(const float *__restrict__ theta, int np,
+                                                                   const float *__restrict__ gb1e,
+                                                                   float *__restrict__ grad) {
+    /* Outputs t: [0, 64) b1, then 64 x 8 E1, then np x 8 emb.  Block b takes
+       outputs [64 b, 64 b + 64); wave s sums the terms k = s, s + 16, ... of
+       each output's sum, and the 16 sums are added in wave order. */
+    __shared__ float part[MLP_WSUM_SPLIT][64];
+    const int col = threadIdx.x & 63, s = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + col;
+    const int nb1 = MLP_H, ne1 = MLP_H * MLP_EMB, nout = nb1 + ne1 + np * MLP_EMB;
+    float acc = 0.f;
+    if (t < nb1) {
+        for (int p = s; p < np; p += MLP_WSUM_SPLIT) acc += gb1e[p * MLP_H + t];
+    } else if (t < nb1 + ne1) {
+        const int q = t - nb1, u = q / MLP_EMB, e = q % MLP_EMB;
+        for (int p = s; p < np; p += MLP_WSUM_SPLIT) acc += gb1e[p * MLP_H + u] * theta[MLP_OFF_EMB + p * MLP_EMB + e];
+    } else if (t < nout) {
+        const int q = t - nb1 - ne1, p = q / MLP_EMB, e = q % MLP_EMB;
+        for (int u = s; u < MLP_H; u += MLP_WSUM_SPLIT) acc += gb1e[p * MLP_H + u] * theta[MLP_OFF_E1 + u * MLP_EMB + e];
     }
+    part[s][col] = acc;
+    __syncthreads();
+    if (s != 0 || t >= nout) return;
+    acc = 0.f;
+    for (int k = 0; k < MLP_WSUM_SPLIT; k++) acc += part[k][col];
+    if (t < nb1)
+        grad[MLP_OFF_B1 + t] += acc;
+    else if (t < nb1 + ne1)
+        grad[MLP_OFF_E1 + t - nb1] += acc;
+    else
+        grad[MLP_OFF_EMB + t - nb1 - ne1] += acc;
 }
 
 __global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ theta, const float *__restrict__ grad,
@@ -488,8 +501,8 @@ static int mlp_gradient(struct gpu_state *g, const int *tiles, int ntiles, float
                        g->d_wpart);
     hipLaunchKernelGGL(k_mlp_wsum, dim3((MLP_WPART + 63) / 64), dim3(64 * MLP_WSUM_SPLIT), 0, g->stream, g->d_wpart,
                        grid, g->d_grad);
-    hipLaunchKernelGGL(k_mlp_chain, dim3((MLP_H + MLP_H * MLP_EMB + np * MLP_EMB + 255) / 256), dim3(256), 0,
-                       g->stream, g->d_theta, np, g->d_gb1e, g->d_grad);
+    hipLaunchKernelGGL(k_mlp_chain, dim3((MLP_H + MLP_H * MLP_EMB + np * MLP_EMB + 63) / 64), dim3(64 * MLP_WSUM_SPLIT),
+                       0, g->stream, g->d_theta, np, g->d_gb1e, g->d_grad);
     HIPCHECK(hipGetLastError());
     return engine_err_ok;
 }
