@@ -557,6 +557,9 @@ int engine_setvelocities(struct engine *e, const double *v, int N);
 int engine_getenergies(struct engine *e, double *energies, long long *counts);
 /* Copy the host sortlist of cell cid, direction sid, into out (count entries). */
 int engine_getsortlist(struct engine *e, int cid, int sid, unsigned int *out);
+/* Every cell's host sortlist block (13*count words, cell.c:443 layout), cells
+   in id order; out == NULL returns the word count. */
+long long engine_getsortlists(struct engine *e, unsigned int *out);
 /* The task list as rows (type, i, j, flags); returns the number of tasks. */
 int engine_gettasks(struct engine *e, int *out, int max);
 /* Per-cell geometry: counts[cid], and the particle ids in cell order (N entries). */

@@ -89,6 +89,7 @@ def library():
         "engine_setvelocities": (C.c_int, [C.c_void_p, _dp, C.c_int]),
         "engine_getenergies": (C.c_int, [C.c_void_p, _dp, _llp]),
         "engine_getsortlist": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up]),
+        "engine_getsortlists": (C.c_longlong, [C.c_void_p, _up]),
         "engine_getcells": (C.c_int, [C.c_void_p, _ip, _ip]),
         "engine_getlocal": (C.c_int, [C.c_void_p, _dp, _ip, C.c_int]),
         "engine_getmaxdx": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -469,6 +470,15 @@ class Engine:
         out = np.zeros(max(count, 1), dtype=np.uint32)
         check(self.lib.engine_getsortlist(self.e, cid, sid, ptr(out, C.c_uint)), "engine_getsortlist")
         return out[:count]
+
+    def sortlists(self):
+        """Every cell's host sortlist block (13*count words), cells in id order."""
+        n = self.lib.engine_getsortlists(self.e, None)
+        if n < 0:
+            check(int(n), "engine_getsortlists")
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        self.lib.engine_getsortlists(self.e, ptr(out, C.c_uint))
+        return out[:n]
 
     # ------------------------------------------------------------ MLP surrogate
 

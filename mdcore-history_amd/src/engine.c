@@ -536,6 +536,18 @@ int engine_getsortlist(struct engine *e, int cid, int sid, unsigned int *out) {
     return c->count;
 }
 
+long long engine_getsortlists(struct engine *e, unsigned int *out) {
+    long long off = 0;
+    int cid;
+    if (e == NULL) return eerror_(engine_err_null);
+    for (cid = 0; cid < e->s.nr_cells; cid++) {
+        struct cell *c = &e->s.cells[cid];
+        if (out != NULL && c->count > 0) memcpy(&out[off], c->sortlist, sizeof(unsigned int) * 13 * c->count);
+        off += 13LL * c->count;
+    }
+    return off;
+}
+
 int engine_gettasks(struct engine *e, int *out, int max) {
     int k;
     if (e == NULL) return eerror_(engine_err_null);

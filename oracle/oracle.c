@@ -75,6 +75,8 @@ struct or_system {
        the last sort (runner.c:773: cells are re-sorted only on a rebuild) */
     double *maxdx;
     int keep_sort;
+    /* pairs evaluated by each task in the last or_nonbond */
+    long long *tcount;
     /* per-thread scratch of or_nonbond, kept between calls */
     int nbuf;
     double *fbuf, *ebuf;
@@ -103,6 +105,37 @@ unsigned int or_key(const double *x, const double *axis, double dscale, double b
     return (unsigned int)(dscale * (bias + x[0] * axis[0] + x[1] * axis[1] + x[2] * axis[2]));
 }
 
+/* Planted-mistake hooks.  The tests compile this file once more with one of
+   these defined and check that the hand-traced tie orders then fail; the
+   oracle itself is always built without them.
+     OR_PLANT_PIVOT_UP  pivot at (lo+hi+1)/2 instead of (lo+hi)/2
+     OR_PLANT_CUT16     selection sort below 16 elements instead of 15
+     OR_PLANT_LAST_MAX  selection sort keeps the last maximum ('>=')
+     OR_PLANT_GE_SCANS  partition scans pass keys equal to the pivot
+                        (bounded to the segment so the variant stays in it) */
+#ifdef OR_PLANT_PIVOT_UP
+#define OR_QS_MID(lo, hi) (((lo) + (hi) + 1) / 2)
+#else
+#define OR_QS_MID(lo, hi) (((lo) + (hi)) / 2)
+#endif
+#ifdef OR_PLANT_CUT16
+#define OR_QS_CUT 16
+#else
+#define OR_QS_CUT 15
+#endif
+#ifdef OR_PLANT_LAST_MAX
+#define OR_QS_BETTER(kj, km) ((kj) >= (km))
+#else
+#define OR_QS_BETTER(kj, km) ((kj) > (km))
+#endif
+#ifdef OR_PLANT_GE_SCANS
+#define OR_QS_SCAN_I (i < hi && (a[i] & 0xffff) >= pivot)
+#define OR_QS_SCAN_J (j > lo && (a[j] & 0xffff) <= pivot)
+#else
+#define OR_QS_SCAN_I ((a[i] & 0xffff) > pivot)
+#define OR_QS_SCAN_J ((a[j] & 0xffff) < pivot)
+#endif
+
 void or_sort_desc(unsigned int *a, int N) {
     struct {
         short lo, hi;
@@ -116,11 +149,11 @@ void or_sort_desc(unsigned int *a, int N) {
         lo = st[qpos].lo;
         hi = st[qpos].hi;
         qpos--;
-        if (hi - lo < 15) {
+        if (hi - lo < OR_QS_CUT) {
             for (i = lo; i < hi; i++) {
                 imax = i;
                 for (j = i + 1; j <= hi; j++)
-                    if ((a[j] & 0xffff) > (a[imax] & 0xffff)) imax = j;
+                    if (OR_QS_BETTER(a[j] & 0xffff, a[imax] & 0xffff)) imax = j;
                 if (imax != i) {
                     t = a[imax];
                     a[imax] = a[i];
@@ -129,12 +162,12 @@ void or_sort_desc(unsigned int *a, int N) {
             }
             continue;
         }
-        pivot = a[(lo + hi) / 2] & 0xffff;
+        pivot = a[OR_QS_MID(lo, hi)] & 0xffff;
         i = lo;
         j = hi;
         while (i <= j) {
-            while ((a[i] & 0xffff) > pivot) i++;
-            while ((a[j] & 0xffff) < pivot) j--;
+            while (OR_QS_SCAN_I) i++;
+            while (OR_QS_SCAN_J) j--;
             if (i <= j) {
                 if (i < j) {
                     t = a[i];
@@ -268,6 +301,7 @@ or_system *or_create(const double *origin, const double *dim, const double *L, d
                             }
                         }
             }
+    s->tcount = (long long *)calloc(s->ntask + 1, sizeof(long long));
 
     /* Tables (copied). */
     s->max_type = tab->max_type;
@@ -453,6 +487,7 @@ void or_free(or_system *s) {
     free(s->ti);
     free(s->tj);
     free(s->tsid);
+    free(s->tcount);
     free(s->pind);
     free(s->tn);
     free(s->toff);
@@ -613,12 +648,14 @@ static void *worker_main(void *arg) {
         return NULL;
     }
     while ((t = __atomic_fetch_add(w->next, 1, __ATOMIC_RELAXED)) < s->ntask) {
+        long long np0 = A.np;
         A.epot_c = 0.0;
         if (s->tt[t] == T_SELF)
             do_self(s, s->ti[t], w->f, w->foff, &A);
         else
             do_pair(s, s->ti[t], s->tj[t], s->tsid[t], w->mode, w->f, w->foff, &A);
         w->cepot[s->ti[t]] += A.epot_c;
+        s->tcount[t] = A.np - np0;
     }
     w->vir = A.vir;
     w->np = A.np;
@@ -868,3 +905,23 @@ int or_task(const or_system *s, int k, int *type, int *i, int *j, int *sid) {
 }
 
 double or_dscale(const or_system *s) { return s->dscale; }
+
+int or_task_counts(const or_system *s, long long *out) {
+    memcpy(out, s->tcount, sizeof(long long) * s->ntask);
+    return s->ntask;
+}
+
+int or_cell_x(const or_system *s, int c, double *x) {
+    if (c < 0 || c >= s->ncells) return -1;
+    memcpy(x, s->cx[c], sizeof(double) * 3 * s->cnt[c]);
+    return s->cnt[c];
+}
+
+long long or_sortlists(const or_system *s, unsigned int *out) {
+    long long off = 0;
+    for (int c = 0; c < s->ncells; c++) {
+        if (out) memcpy(&out[off], s->sl[c], sizeof(unsigned int) * 13 * s->cnt[c]);
+        off += 13LL * s->cnt[c];
+    }
+    return off;
+}

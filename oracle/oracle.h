@@ -65,6 +65,14 @@ int or_sortlist(const or_system *s, int cid, int sid, unsigned int *out);
 int or_ntasks(const or_system *s, int *npair, int *nself);
 int or_task(const or_system *s, int k, int *type, int *i, int *j, int *sid);
 double or_dscale(const or_system *s);
+/* Pairs evaluated by each task (or_task order) in the last or_nonbond. */
+int or_task_counts(const or_system *s, long long *out);
+/* Cell-relative positions of cell c in storage order (3 per particle). */
+int or_cell_x(const or_system *s, int c, double *x);
+/* Every cell's 13 sort lists, cells in id order, each cell's block laid out
+   as cell.c:443 does (direction sid at sid*count); directions not flagged
+   hold whatever the last sort left.  out == NULL returns the word count. */
+long long or_sortlists(const or_system *s, unsigned int *out);
 
 /* Building blocks, exported for unit tests. */
 void or_sort_desc(unsigned int *a, int N);

@@ -56,6 +56,9 @@ def lib():
             "or_ntasks": (C.c_int, [C.c_void_p, _ip, _ip]),
             "or_task": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip]),
             "or_dscale": (C.c_double, [C.c_void_p]),
+            "or_task_counts": (C.c_int, [C.c_void_p, _llp]),
+            "or_cell_x": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+            "or_sortlists": (C.c_longlong, [C.c_void_p, _up]),
             "or_set_verlet": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
             "or_sort_desc": (None, [_up, C.c_int]),
             "or_key": (C.c_uint, [_dp, _dp, C.c_double, C.c_double]),
@@ -220,6 +223,26 @@ class Oracle:
 
     def dscale(self):
         return lib().or_dscale(self.h)
+
+    def task_counts(self):
+        """Pairs evaluated by each task (tasks() order) in the last nonbond()."""
+        nt = lib().or_ntasks(self.h, None, None)
+        out = np.zeros(max(nt, 1), np.int64)
+        lib().or_task_counts(self.h, _p(out, C.c_longlong))
+        return out[:nt]
+
+    def cell_x(self, cid, count):
+        """Cell-relative FP64 positions of cell cid in storage order."""
+        out = np.zeros((max(count, 1), 3))
+        lib().or_cell_x(self.h, cid, _p(out))
+        return out[:count]
+
+    def sortlists(self):
+        """All cells' 13 sort lists, cells in id order (13*count words each)."""
+        n = lib().or_sortlists(self.h, None)
+        out = np.zeros(max(n, 1), np.uint32)
+        lib().or_sortlists(self.h, _p(out, C.c_uint))
+        return out[:n]
 
     def close(self):
         if self.h:

@@ -1,7 +1,8 @@
 """GPU path vs the FP64 oracle on the same inputs.
 
-Bit-exact: cell contents, every sorted list, pair counts (both windows),
-exclusion counts.  Floating point: per-atom force relative RMSE <= 1e-5 and
+Bit-exact: cell contents, every sorted list, pair counts (both windows,
+including pair tasks that the reference window's unsigned break skips
+whole), exclusion counts.  Floating point: per-atom force relative RMSE <= 1e-5 and
 energy / virial relative error <= 1e-6 against the FP64 oracle (FP32 pair
 arithmetic, two-sum accumulation, FP64 reductions).  Every system runs with
 both pair-kernel shells (full: each pair at both cells; half: each pair task
@@ -15,6 +16,7 @@ import pytest
 import mdcore as md
 import pyoracle as po
 import systems
+from cases import face_pairs
 from gpuutil import engine_pass, rel_rmse
 
 pytestmark = pytest.mark.gpu
@@ -39,6 +41,9 @@ SYSTEMS = {
     # simple cubic lattice, 64 atoms per cell: every direction's keys repeat
     # many times, so nearly every sort list goes through the tie sort
     "lattice_ties": lambda: lattice_ties(),
+    # pairs planted across cell faces: the reference window's unsigned break
+    # wraps and skips whole pair tasks that hold in-range pairs
+    "face_pairs": lambda: face_pairs(),
 }
 
 

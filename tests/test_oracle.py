@@ -2,11 +2,18 @@
 
 - direction tables: bit-exact vs the reference's compiled cell.c, and the
   geometry they encode (unit axis of each flipped offset);
-- sort: hand-traced tie orders of the reference quicksort/selection sort,
-  plus sortedness and permutation on random keys;
+- sort: tie orders of the reference quicksort/selection sort traced by hand
+  (all-equal lists and three mixed repeated-key lists of 16, 20 and 24
+  words, one with a nested partition), each planted mistake in the oracle's
+  quicksort failing at least one trace; sortedness and permutation on
+  random keys;
 - sort keys: range and spacing that the constants bias/dscale fix;
 - task list: identical to the host library's, and the counts of the
   1M-atom geometry (62 pair tasks per cell);
+- per-task window counts: the sorted window's visited set does not depend
+  on the order of equal keys, so every task's count has a closed form over
+  the keys, the shift and the cutoff, checked task by task on span-2
+  geometries in both window modes and with widened windows;
 - pair loops: all-pairs brute force (the projected window finds every pair,
   the reference window misses some on span-2 cells), Newton's third law,
   translation invariance, threaded == serial, exact exclusion cancellation;
@@ -15,6 +22,10 @@
   does not.
 """
 import ctypes as C
+import math
+import shutil
+import subprocess
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -22,6 +33,7 @@ import pytest
 import mdcore as md
 import pyoracle as po
 import systems
+from cases import face_pairs
 
 # ---------------------------------------------------------------- directions
 
@@ -86,6 +98,70 @@ def test_quicksort_equal_keys_reverse_hand_traced(n):
     # swaps equal keys.  Result: the indices reversed.
     out = po.sort_desc(words([9] * n))
     assert list(out >> 16) == list(range(n - 1, -1, -1))
+
+
+# Mixed repeated keys, traced by hand through runner.c:110-174 (pivot =
+# key at (lo+hi)/2; scan i up while key > pivot, j down while key < pivot;
+# swap when i < j, then step both; push [lo,j] and [i,hi]; selection sort
+# keeping the first maximum when hi-lo < 15).  Positions are p0..; entries
+# are (index, key).
+#
+# A, n=16: pivot key 5 (index 7).  Swaps p0/p15, p1/p13, p2/p11, p4/p9,
+#   p5/p7; i=j=6 holds a 5 and is stepped over: i=7, j=5.  Left p0..p5 =
+#   (15,6)(13,8)(11,5)(3,8)(9,7)(7,5); p6 = (6,5); right p7..p15 =
+#   (5,1)(8,2)(4,5)(10,3)(2,3)(12,0)(1,5)(14,3)(0,3).  Both sides go to
+#   selection sort: left -> 13 3 9 15 11 7; right -> 4 1 10 2 14 0 8 5 12.
+# B, n=20: pivot key 2 (index 9).  Swaps p0/p19, p1/p18, p3/p17, p4/p15,
+#   p5/p14, p7/p13, p8/p11, p9/p10 -> i=10, j=9.  Left p0..p9 =
+#   19 18 2 17 15 14 6 13 11 10, right p10..p19 = 9 8 12 7 5 4 16 3 1 0;
+#   selection sorts give 18 2 14 6 10 19 17 13 11 15 and
+#   9 7 4 3 0 12 16 8 1 5.
+# C, n=24: pivot key 2 (index 11).  Swaps p11/p23, p17/p22; i=j=21 holds
+#   a 2: i=22, j=20.  [22,23] = (17,1)(11,2) -> 11 17.  [0,20] has 21
+#   words: pivot key 6 at p10 (index 10); swaps p1/p20, p4/p18, p5/p17,
+#   p9/p16, p10/p15 -> i=14, j=13.  [14,20] = 14 10 9 5 4 19 1 ->
+#   10 4 9 19 14 5 1; [0,13] = 0 20 2 3 18 22 6 7 8 16 15 23 12 13 ->
+#   0 3 8 15 6 20 13 7 2 23 12 18 22 16.
+TRACED = {
+    "A": ([3, 5, 3, 8, 5, 1, 5, 5, 2, 7, 3, 5, 0, 8, 3, 6],
+          [13, 3, 9, 15, 11, 7, 6, 4, 1, 10, 2, 14, 0, 8, 5, 12]),
+    "B": ([2, 1, 3, 2, 2, 1, 3, 2, 1, 2, 3, 2, 1, 2, 3, 2, 1, 2, 3, 2],
+          [18, 2, 14, 6, 10, 19, 17, 13, 11, 15, 9, 7, 4, 3, 0, 12, 16, 8, 1, 5]),
+    "C": ([9, 4, 7, 9, 6, 4, 8, 7, 9, 5, 6, 2, 7, 8, 4, 9, 6, 1, 7, 5, 8, 2, 6, 7],
+          [0, 3, 8, 15, 6, 20, 13, 7, 2, 23, 12, 18, 22, 16, 10, 4, 9, 19, 14, 5, 1, 21, 11, 17]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(TRACED))
+def test_quicksort_mixed_keys_hand_traced(case):
+    keys, order = TRACED[case]
+    out = po.sort_desc(words(keys))
+    assert list(out >> 16) == order
+    assert list(out & 0xFFFF) == [keys[i] for i in order]
+
+
+PLANTS = ["OR_PLANT_PIVOT_UP", "OR_PLANT_CUT16", "OR_PLANT_LAST_MAX", "OR_PLANT_GE_SCANS"]
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+@pytest.mark.parametrize("plant", PLANTS)
+def test_planted_sort_mistakes_fail_the_traces(plant, tmp_path):
+    """The oracle compiled with one planted mistake (pivot rounded up, the
+    selection-sort threshold at 16, the last maximum kept, scans that pass
+    keys equal to the pivot) gives another order for at least one traced
+    list: the traces pin the tie order, not just sortedness."""
+    so = tmp_path / f"lib_{plant}.so"
+    src = Path(po.HERE) / "oracle.c"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", f"-D{plant}", "-o", str(so), str(src),
+                    "-lm", "-lpthread"], check=True)
+    lib = C.CDLL(str(so))
+    lib.or_sort_desc.argtypes = [C.POINTER(C.c_uint), C.c_int]
+    wrong = 0
+    for keys, order in TRACED.values():
+        a = words(keys).copy()
+        lib.or_sort_desc(a.ctypes.data_as(C.POINTER(C.c_uint)), len(a))
+        wrong += list(a >> 16) != order
+    assert wrong >= 1, plant
 
 
 @pytest.mark.parametrize("seed", range(5))
@@ -219,23 +295,139 @@ def test_exclusion_cancels_the_pair():
     assert np.abs(r["f"]).max() <= 1e-12 * 1e4
 
 
-def test_reference_window_miss_fraction_matches_probe():
-    """The |shift| window of runner_dopair.c:451-452 drops in-cutoff pairs on
-    span-2 offsets.  At the 1M-mixture geometry (0.6 nm cells, rc 1.2 nm,
-    100 atoms/nm^3, uniform random positions) the compiled FP64 reference
-    runner was measured to miss 2.186 % of the pairs the exact window finds;
-    the restatement must land on the same figure (sampling noise ~0.02 %)."""
-    L = 6.0
-    N = int(100 * L ** 3)
-    x = md.gen_uniform(3 * N, 0.0, L, 1).reshape(-1, 3)
-    pot = md.LJ126(0.05, 1.2, 1e-6, 1e-3, 1e-3)
-    sysm = systems.System("uniform", np.zeros(3), np.full(3, L), np.full(3, 0.6), 1.2, x, np.ones(N, np.int32),
+def window_counts_closed_form(o, sysm, mode, maxdx=None):
+    """Pairs every task of the oracle's task list evaluates, from a formula
+    that does not depend on the order of equal keys.
+
+    Pair task (ci, cj, sid) (runner_dopair.c:397-537): i walks ci's list in
+    descending key order, j walks cj's in ascending order while
+    key_j + dnshift - key_i < dmaxdist, in unsigned arithmetic for the
+    reference window (dnshift = dscale*|shift|) and signed for the projected
+    one (dscale*shift.n).  The break on i (line 462) removes only atoms for
+    which no j qualifies, except when it wraps: with key_i,max > key_j,min +
+    dnshift the unsigned difference is huge at the first i and the task is
+    empty.  The visited set is therefore {(i, j): key_j + dnshift - key_i <
+    dmaxdist} (empty on a wrap), of which the pairs with a potential and
+    r^2 <= rc^2 are evaluated, r^2 from dx = (x_i - shift) - x_j
+    (fptype.h:275-278).  Self task (753-855): every j < i in storage order.
+    Keys: (unsigned)(dscale*(bias + x.n)) (runner_dosort.c:148-149), from
+    the cell-relative FP64 positions."""
+    dim = np.asarray(sysm.dim, float)
+    L = np.asarray(sysm.L, float)
+    cdim = np.floor(dim / L).astype(int)
+    h = dim / cdim
+    origin = np.asarray(sysm.origin, float)
+    bias = math.sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2])
+    dscale = 32767.0 / (2 * bias)
+    assert dscale == o.dscale()
+    rc = float(sysm.cutoff)
+    rc2 = rc * rc
+    axes = np.ctypeslib.as_array((C.c_double * 39).in_dll(po.lib(), "or_axis")).reshape(13, 3)
+    tabs = systems.tables(sysm)
+    mt = tabs["max_type"]
+    haspot = (tabs["pind"] >= 0).reshape(mt, mt)
+    if maxdx is None:
+        maxdx = np.zeros(o.ncells)
+    cells = []
+    for cid in range(o.ncells):
+        ids, _ = o.cell(cid)
+        cells.append((o.cell_x(cid, len(ids)), sysm.types[ids]))
+    loc = np.stack(np.unravel_index(np.arange(o.ncells), tuple(cdim)), 1).astype(float)
+    corg = origin + loc * h
+
+    def keys(x, sid):
+        n = axes[sid]
+        return (dscale * (((bias + x[:, 0] * n[0]) + x[:, 1] * n[1]) + x[:, 2] * n[2])).astype(np.int64)
+
+    out = []
+    wraps = 0
+    wrapped_pairs = 0
+    for typ, ci, cj, sid in o.tasks():
+        xi, ti = cells[ci]
+        if typ == 1:
+            d = xi[:, None, :] - xi[None, :, :]
+            r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+            ok = np.tril(np.ones((len(xi), len(xi)), bool), -1) & (r2 <= rc2) & haspot[ti[:, None], ti[None, :]]
+            out.append(int(ok.sum()))
+            continue
+        xj, tj = cells[cj]
+        if len(xi) == 0 or len(xj) == 0:
+            out.append(0)
+            continue
+        sh = corg[cj] - corg[ci]
+        sh = np.where(sh * 2 > dim, sh - dim, np.where(sh * 2 < -dim, sh + dim, sh))
+        dmaxdist = int(2 + dscale * ((rc + maxdx[ci]) + maxdx[cj]))
+        if mode == po.OR_WINDOW_REF:
+            dnshift = int(dscale * math.sqrt(sh[0] * sh[0] + sh[1] * sh[1] + sh[2] * sh[2]))
+        else:
+            n = axes[sid]
+            dnshift = int(dscale * ((sh[0] * n[0] + sh[1] * n[1]) + sh[2] * n[2]))
+        ki, kj = keys(xi, sid), keys(xj, sid)
+        d = (xi - sh)[:, None, :] - xj[None, :, :]
+        r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        ok = (kj[None, :] + dnshift - ki[:, None] < dmaxdist) & (r2 <= rc2) & haspot[ti[:, None], tj[None, :]]
+        if mode == po.OR_WINDOW_REF and ki.max() > kj.min() + dnshift:
+            wraps += 1
+            wrapped_pairs += int(ok.sum())
+            out.append(0)
+        else:
+            out.append(int(ok.sum()))
+    return np.array(out, np.int64), wraps, wrapped_pairs
+
+
+def uniform_system(dim, L, cutoff, density, seed):
+    dim = np.asarray(dim, float)
+    N = int(density * np.prod(dim))
+    x = md.gen_uniform(3 * N, 0.0, 1.0, seed).reshape(-1, 3) * dim
+    pot = md.LJ126(0.05, cutoff, 1e-6, 1e-3, 1e-3)
+    return systems.System("uniform", np.zeros(3), dim, np.asarray(L, float), cutoff, x, np.ones(N, np.int32),
                           np.array([0.0, 1.0]), [(1, 1, pot)])
+
+
+WINDOW_GEOMETRIES = {
+    # the 1M mixture's geometry: 0.6 nm cells, rc 1.2 nm (span 2 on every axis)
+    "mixture_span2": lambda: systems.mixture(5000, seed=4),
+    # span (2,1,1): offsets such as (2,1,0), where |shift| and shift.n differ most
+    "aniso_span211": lambda: uniform_system((4.4, 5.25, 5.25), (0.55, 1.05, 1.05), 1.0, 60.0, 3),
+    # pairs planted across faces, edges and corners: the unsigned break wraps
+    "face_pairs": lambda: face_pairs(),
+}
+
+
+@pytest.mark.parametrize("mode", [po.OR_WINDOW_REF, po.OR_WINDOW_PROJ], ids=["ref", "proj"])
+@pytest.mark.parametrize("geom", sorted(WINDOW_GEOMETRIES))
+def test_window_counts_per_task_closed_form(geom, mode):
+    sysm = WINDOW_GEOMETRIES[geom]()
     o = po.from_system(sysm)
-    ref = o.nonbond(po.OR_WINDOW_REF, 4)["pairs"]
-    prj = o.nonbond(po.OR_WINDOW_PROJ, 4)["pairs"]
-    miss = 1.0 - ref / prj
-    assert 0.0210 <= miss <= 0.0228, miss
+    got = o.nonbond(mode)
+    counts = o.task_counts()
+    want, wraps, wrapped = window_counts_closed_form(o, sysm, mode)
+    assert counts.sum() == got["pairs"]
+    assert np.array_equal(counts, want)
+    if mode == po.OR_WINDOW_REF:
+        if geom == "face_pairs":
+            # the wrap-around rule is exercised: tasks it empties hold pairs
+            # a signed window would have evaluated
+            assert wraps > 0 and wrapped > 0
+        prj = po.from_system(sysm).nonbond(po.OR_WINDOW_PROJ)["pairs"]
+        print(f"{geom}: reference window misses {1 - got['pairs'] / prj:.4%} of the in-cutoff pairs; "
+              f"{wraps} wrapped tasks holding {wrapped} pairs")
+    o.close()
+
+
+def test_window_counts_widened_closed_form():
+    """Per-cell maxdx widens dmaxdist = 2 + dscale*(rc + maxdx_i + maxdx_j)
+    (runner_dopair.c:428) with a fresh sort: every task's count still
+    follows the closed form."""
+    sysm = WINDOW_GEOMETRIES["mixture_span2"]()
+    o = po.from_system(sysm)
+    maxdx = np.random.default_rng(5).uniform(0.0, 0.06, o.ncells)
+    o.set_verlet(None, maxdx, keep_sort=False)
+    for mode in (po.OR_WINDOW_REF, po.OR_WINDOW_PROJ):
+        o.nonbond(mode)
+        want, _, _ = window_counts_closed_form(o, sysm, mode, maxdx)
+        assert np.array_equal(o.task_counts(), want), mode
+    o.close()
 
 
 def test_sort_reuse_window_widening():
@@ -275,3 +467,27 @@ def test_sort_reuse_window_widening():
     assert fresh["pairs"] == bf["pairs"]
     print(f"pairs: brute force {bf['pairs']}, reused sort widened {reuse['pairs']}, "
           f"not widened {narrow['pairs']}")
+
+
+def test_bulk_sortlists_match_per_cell_lists():
+    """or_sortlists (the oracle's every-cell copy used by the full-size GPU
+    comparison) holds each cell's flagged lists at sid*count in its block,
+    cells in id order, and the host library's engine_getsortlists has the
+    same size."""
+    sysm = systems.mixture(3000, seed=2)
+    o = po.from_system(sysm)
+    o.sort()
+    bulk = o.sortlists()
+    off = 0
+    for cid in range(o.ncells):
+        ids, flags = o.cell(cid)
+        n = len(ids)
+        for sid in range(13):
+            if n and (flags >> sid) & 1:
+                assert np.array_equal(bulk[off + sid * n:off + (sid + 1) * n], o.sortlist(cid, sid, n))
+        off += 13 * n
+    assert off == len(bulk) == 13 * sysm.N
+    eng = systems.build_engine(sysm, md.engine_flag_gpu, start=False)
+    assert len(eng.sortlists()) == 13 * sysm.N
+    eng.close()
+    o.close()
