@@ -133,6 +133,13 @@ struct potential *potential_create_harmonic(double a, double b, double K, double
    for GROMACS units); eps_rf <= 0 means eps_rf = infinity. */
 struct potential *potential_create_LJ126_RF(double a, double b, double A, double B, double q,
                                             double ke, double eps_rf, double tol);
+/* New, like potential_create_LJ126_RF, with the Lennard-Jones part switched off
+   between s and b by the reference's potential_switch (potential.c:60-96):
+     E(r) = (A/r^12 - B/r^6) S(r) + ke*q*(1/r + krf*r^2 - crf).
+   With eps_rf <= 0 (conducting boundary) E and dE/dr both vanish at b, so
+   the table is force-continuous at the cutoff. */
+struct potential *potential_create_LJ126_switch_RF(double a, double b, double A, double B, double s, double q,
+                                                   double ke, double eps_rf, double tol);
 
 /* Host FP64 evaluation of a table (potential_eval.h:67-104): e and f=(dE/dr)/r. */
 void potential_eval(const struct potential *p, FPTYPE r2, FPTYPE *e, FPTYPE *f);

@@ -124,6 +124,7 @@ struct gpu_state {
     int mlp_np, mlp_nparams;
     long long mlp_step;
     float *d_theta, *d_grad, *d_adam_m, *d_adam_v, *d_gb1e, *d_b1e, *d_loss, *d_wpart;
+    float *d_tpart, *d_err; /* per-tile gradient rows; bad-pair flag */
     mlp_bf16x8 *d_w2f, *d_w2b;
     float4 *d_msc;
     float *d_sr, *d_sE, *d_sF;

@@ -9,11 +9,18 @@
  *                  (dH1 = W2^T dZ2 and dW2 = dZ2 H1^T) run on
  *                  v_mfma_f32_16x16x32_bf16 with bf16 operands and fp32
  *                  accumulation; the rest is fp32 VALU.
+ *   k_mlp_wsum     fixed-order sum of the per-wave partial rows of the
+ *                  shared weights (W2, W1, b2);
+ *   k_mlp_tsum     fixed-order sum of the per-tile rows of the per-pair
+ *                  parameters (heads W3, b3, the bias gradient of b1e) and
+ *                  of the loss, tiles in batch order;
  *   k_mlp_chain    gradient of b1, E1 and the embeddings from the gradient
  *                  of the per-pair bias b1e = b1 + E1 emb[p];
  *   k_mlp_adam     Adam on the fp32 master parameters;
  *   k_mlp_publish  derived device operands: b1e and the bf16 W2 operands in
  *                  the lane order of the MFMA tiles.
+ * Every gradient sum runs in a fixed order (no float atomics), so a
+ * gradient, and a whole training run, gives the same bits every time.
  * k_mlp_eval evaluates the network on arbitrary (r, pair) samples.
  * The forward pass and the lane layout are in mlp_tile.h.
  */
@@ -27,6 +34,14 @@
 
 #define MLP_WPART (MLP_H * MLP_H + 3 * MLP_H) /* per-wave partial: W2, W1, b2 */
 #define MLP_GRID_MAX 2048
+/* Per-tile row: [0,128) dW3 of the tile's pair (energy head, then force
+   head), [128,130) db3, [130,194) gradient of the pair's b1e, [194] the
+   tile's loss (times 1/n), [195] padding. */
+#define MLP_TROW 196
+#define MLP_TR_B3 128
+#define MLP_TR_B1E 130
+#define MLP_TR_LOSS 194
+#define MLP_TCHUNK 4096 /* tiles per gradient launch (rows of the per-tile buffer) */
 
 /* ------------------------------------------------------------ kernels */
 
@@ -34,6 +49,7 @@ __global__ __launch_bounds__(256) void k_mlp_fill(float *__restrict__ a, long lo
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
         a[k] = v;
 }
+
 
 /* K index carried by element e of lane group g in K-block kb. */
 __device__ __forceinline__ int mlp_kperm(int kb, int g, int e) { return 16 * (2 * kb + (e >> 2)) + 4 * g + (e & 3); }
@@ -74,13 +90,15 @@ __device__ __forceinline__ float mlp_sum16(float v) {
    the dW2 product).  Samples of tile t are s = 16 t + n, n = 0..15, and all
    16 share species pair sp[16 t].  Loss per sample:
    wE (y0 - asinh(E/sE))^2 + wF (y1 - asinh(F/sF))^2; gradients are of the
-   sum times inv_n. */
+   sum times inv_n.  Sums over the shared weights go to this wave's partial
+   row, those of the tile's pair to the tile's row of tpart (index ti of
+   tiles[]).  A pair id outside [0, np) is read as pair 0 and sets *err. */
 __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *__restrict__ sr,
                                                  const int *__restrict__ sp, const float *__restrict__ tE,
                                                  const float *__restrict__ tF, const int *__restrict__ tiles,
                                                  int ntiles, float wE, float wF, float inv_n, int np,
-                                                 float *__restrict__ grad, float *__restrict__ gb1e,
-                                                 float *__restrict__ loss, float *__restrict__ partial) {
+                                                 float *__restrict__ tpart, float *__restrict__ partial,
+                                                 float *__restrict__ err) {
     __shared__ __attribute__((aligned(16))) __bf16 h1t[MLP_H][32]; /* [in unit][sample of the tile pair] */
     __shared__ __attribute__((aligned(16))) __bf16 dzt[MLP_H][32]; /* [out unit][sample]               */
     const int lane = threadIdx.x, n = lane & 15, g = lane >> 4;
@@ -91,7 +109,6 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
         dw2[q] = (mlp_f4){0.f, 0.f, 0.f, 0.f};
         dw1a[q] = dw1b[q] = db2[q] = 0.f;
     }
-    float lsum = 0.f;
     const int ntp = (ntiles + 1) / 2;
     for (int tp = blockIdx.x; tp < ntp; tp += gridDim.x) {
         for (int sub = 0; sub < 2; sub++) {
@@ -99,14 +116,22 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
             const bool valid = ti < ntiles;
             const int tile = tiles[valid ? ti : 0];
             const int s = 16 * tile + n;
-            const int p = sp[16 * tile];
+            int p = sp[16 * tile];
+            if ((unsigned)p >= (unsigned)np) {
+                if (lane == 0) *err = 1.0f;
+                p = 0;
+            }
+            float *trow = tpart + (size_t)ti * MLP_TROW;
             const float r = sr[s];
             float z1[16], z2[16], h2[16], y0, y1;
             mlp_bf16x8 h1b[2];
             mlp_forward16(N, r, p, lane, z1, h1b, z2, h2, y0, y1);
             const float4 sc = N.sc[p];
             const float d0 = y0 - asinhf(tE[s] / sc.z), d1 = y1 - asinhf(tF[s] / sc.w);
-            if (valid && g == 0) lsum += wE * d0 * d0 + wF * d1 * d1;
+            {
+                const float lt = mlp_sum16(valid && g == 0 ? wE * d0 * d0 + wF * d1 * d1 : 0.f);
+                if (valid && lane == 0) trow[MLP_TR_LOSS] = lt * inv_n;
+            }
             const float dy0 = valid ? 2.0f * wE * d0 * inv_n : 0.0f;
             const float dy1 = valid ? 2.0f * wF * d1 * inv_n : 0.0f;
 
@@ -122,15 +147,15 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
                 const float ga = mlp_sum16(dy0 * h2[j]);
                 const float gb = mlp_sum16(dy1 * h2[j]);
                 if (valid && n == 0) {
-                    atomicAdd(&grad[MLP_OFF_W3(np) + p * 2 * MLP_H + u], ga);
-                    atomicAdd(&grad[MLP_OFF_W3(np) + p * 2 * MLP_H + MLP_H + u], gb);
+                    trow[u] = ga;
+                    trow[MLP_H + u] = gb;
                 }
             }
             {
                 const float ga = mlp_sum16(dy0), gb = mlp_sum16(dy1);
                 if (valid && lane == 0) {
-                    atomicAdd(&grad[MLP_OFF_B3(np) + 2 * p], ga);
-                    atomicAdd(&grad[MLP_OFF_B3(np) + 2 * p + 1], gb);
+                    trow[MLP_TR_B3] = ga;
+                    trow[MLP_TR_B3 + 1] = gb;
                 }
             }
 
@@ -154,7 +179,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
                     dw1a[j] += dz1 * rn;
                     dw1b[j] += dz1 * irn;
                     const float gsum = mlp_sum16(dz1);
-                    if (valid && n == 0) atomicAdd(&gb1e[p * MLP_H + mlp_unit(j, lane)], gsum);
+                    if (valid && n == 0) trow[MLP_TR_B1E + mlp_unit(j, lane)] = gsum;
                 }
             }
 
@@ -199,8 +224,6 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
             row[MLP_H * MLP_H + 2 * MLP_H + u] = c;
         }
     }
-    lsum = mlp_sum16(lsum);
-    if (lane == 0) atomicAdd(loss, lsum * inv_n);
 }
 
 /* Sum the per-wave partial rows of k_mlp_grad into the gradient.  Block b
@@ -228,6 +251,45 @@ __global__ __launch_bounds__(64 * MLP_WSUM_SPLIT) void k_mlp_wsum(const float *_
         grad[MLP_OFF_W1 + t - MLP_H * MLP_H] += acc;
     else
         grad[MLP_OFF_B2 + t - MLP_H * MLP_H - 2 * MLP_H] += acc;
+}
+
+/* Add the per-tile rows of k_mlp_grad: block p < np sums the rows of the
+   tiles of pair p, in tile order, into the gradients of its heads and
+   b1e; block np sums the tiles' losses (thread k takes tiles k, k + 256,
+   ... in order, then a fixed tree).  A pair id outside [0, np) marks no
+   tile; with *err set the loss becomes NaN. */
+__global__ __launch_bounds__(256) void k_mlp_tsum(const float *__restrict__ tpart, const int *__restrict__ tiles,
+                                                  int ntiles, const int *__restrict__ sp, int np,
+                                                  float *__restrict__ grad, float *__restrict__ gb1e,
+                                                  float *__restrict__ loss, const float *__restrict__ err) {
+    __shared__ unsigned char mine[MLP_TCHUNK];
+    __shared__ float red[256];
+    const int p = blockIdx.x, c = threadIdx.x;
+    if (p < np) {
+        for (int t = c; t < ntiles; t += 256) mine[t] = sp[16 * tiles[t]] == p;
+        __syncthreads();
+        if (c < MLP_TR_LOSS) {
+            float acc = 0.f;
+            for (int t = 0; t < ntiles; t++)
+                if (mine[t]) acc += tpart[(size_t)t * MLP_TROW + c];
+            if (c < MLP_TR_B3)
+                grad[MLP_OFF_W3(np) + p * 2 * MLP_H + c] += acc;
+            else if (c < MLP_TR_B1E)
+                grad[MLP_OFF_B3(np) + 2 * p + c - MLP_TR_B3] += acc;
+            else
+                gb1e[p * MLP_H + c - MLP_TR_B1E] += acc;
+        }
+    } else {
+        float acc = 0.f;
+        for (int t = c; t < ntiles; t += 256) acc += tpart[(size_t)t * MLP_TROW + MLP_TR_LOSS];
+        red[c] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (c < w) red[c] += red[c + w];
+            __syncthreads();
+        }
+        if (c == 0) *loss = *err != 0.f ? __builtin_nanf("") : *loss + red[0];
+    }
 }
 
 /* b1e[p][u] = b1[u] + sum_e E1[u][e] emb[p][e]: gradients of b1, E1, emb,
@@ -325,7 +387,9 @@ void gpu_mlp_free(struct gpu_state *g) {
     (void)hipFree(g->d_w2b);
     (void)hipFree(g->d_msc);
     (void)hipFree(g->d_wpart);
-    g->d_wpart = NULL;
+    (void)hipFree(g->d_tpart);
+    (void)hipFree(g->d_err);
+    g->d_wpart = g->d_tpart = g->d_err = NULL;
     (void)hipFree(g->d_sr);
     (void)hipFree(g->d_sp);
     (void)hipFree(g->d_sE);
@@ -447,6 +511,8 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
         hipMalloc(&g->d_w2b, sizeof(mlp_bf16x8) * 512) != hipSuccess ||
         hipMalloc(&g->d_msc, sizeof(float4) * np) != hipSuccess ||
         hipMalloc(&g->d_wpart, sizeof(float) * MLP_WPART * MLP_GRID_MAX) != hipSuccess ||
+        hipMalloc(&g->d_tpart, sizeof(float) * MLP_TROW * MLP_TCHUNK) != hipSuccess ||
+        hipMalloc(&g->d_err, sizeof(float)) != hipSuccess ||
         hipMemcpy(g->d_theta, theta, sizeof(float) * nparams, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(g->d_msc, sc, sizeof(float4) * np, hipMemcpyHostToDevice) != hipSuccess) {
         free(theta);
@@ -458,6 +524,7 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
     free(sc);
     hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_adam_m, (long long)nparams, 0.0f);
     hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_adam_v, (long long)nparams, 0.0f);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(1), dim3(64), 0, g->stream, g->d_err, 1LL, 0.0f);
     if (mlp_publish(g) < 0) return engine_err_gpu;
     HIPCHECK(hipStreamSynchronize(g->stream));
     return nparams;
@@ -486,25 +553,45 @@ static int mlp_check(const struct gpu_state *g, const float *r, const int *pair,
     return engine_err_ok;
 }
 
+/* Buffers of one gradient evaluation: the gradient and loss outputs and the
+   scratch (b1e gradient, per-wave and per-tile partial rows, bad-pair flag),
+   and the largest grid of the k_mlp_grad launches. */
+struct mlp_bufs {
+    float *grad, *loss, *gb1e, *wpart, *tpart, *err;
+    int grid_max;
+};
+
 /* One gradient evaluation over the tiles tiles[0..ntiles) (device array) of
-   the uploaded samples: zero the gradients, k_mlp_grad, k_mlp_chain.  The
-   mean loss adds into *loss (device). */
-static int mlp_gradient(struct gpu_state *g, const int *tiles, int ntiles, float wE, float wF, float *loss) {
+   the samples (device arrays sr, sp, sE, sF): zero the gradients, then per
+   chunk of up to MLP_TCHUNK tiles k_mlp_grad, k_mlp_wsum and k_mlp_tsum,
+   then k_mlp_chain.  The mean loss adds into *B->loss. */
+static int mlp_gradient(struct gpu_state *g, const struct mlp_bufs *B, const float *sr, const int *sp,
+                        const float *sE, const float *sF, const int *tiles, int ntiles, float wE, float wF) {
     const int np = g->mlp_np;
-    const int ntp = (ntiles + 1) / 2;
-    const int grid = ntp < MLP_GRID_MAX ? ntp : MLP_GRID_MAX;
+    const float inv_n = 1.0f / (16.0f * ntiles);
     if (ntiles < 1) return gerror_(engine_err_range, "empty batch");
-    hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_grad, (long long)g->mlp_nparams, 0.0f);
-    hipLaunchKernelGGL(k_mlp_fill, dim3(16), dim3(256), 0, g->stream, g->d_gb1e, (long long)np * MLP_H, 0.0f);
-    hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64), 0, g->stream, gpu_mlp_net(g), g->d_sr, g->d_sp, g->d_sE,
-                       g->d_sF, tiles, ntiles, wE, wF, 1.0f / (16.0f * ntiles), np, g->d_grad, g->d_gb1e, loss,
-                       g->d_wpart);
-    hipLaunchKernelGGL(k_mlp_wsum, dim3((MLP_WPART + 63) / 64), dim3(64 * MLP_WSUM_SPLIT), 0, g->stream, g->d_wpart,
-                       grid, g->d_grad);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, B->grad, (long long)g->mlp_nparams, 0.0f);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(16), dim3(256), 0, g->stream, B->gb1e, (long long)np * MLP_H, 0.0f);
+    for (int c0 = 0; c0 < ntiles; c0 += MLP_TCHUNK) {
+        const int nt = ntiles - c0 < MLP_TCHUNK ? ntiles - c0 : MLP_TCHUNK;
+        const int ntp = (nt + 1) / 2;
+        const int grid = ntp < B->grid_max ? ntp : B->grid_max;
+        hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64), 0, g->stream, gpu_mlp_net(g), sr, sp, sE, sF, tiles + c0,
+                           nt, wE, wF, inv_n, np, B->tpart, B->wpart, B->err);
+        hipLaunchKernelGGL(k_mlp_wsum, dim3((MLP_WPART + 63) / 64), dim3(64 * MLP_WSUM_SPLIT), 0, g->stream,
+                           B->wpart, grid, B->grad);
+        hipLaunchKernelGGL(k_mlp_tsum, dim3(np + 1), dim3(256), 0, g->stream, B->tpart, tiles + c0, nt, sp, np, B->grad,
+                           B->gb1e, B->loss, B->err);
+    }
     hipLaunchKernelGGL(k_mlp_chain, dim3((MLP_H + MLP_H * MLP_EMB + np * MLP_EMB + 63) / 64), dim3(64 * MLP_WSUM_SPLIT),
-                       0, g->stream, g->d_theta, np, g->d_gb1e, g->d_grad);
+                       0, g->stream, g->d_theta, np, B->gb1e, B->grad);
     HIPCHECK(hipGetLastError());
     return engine_err_ok;
+}
+
+static struct mlp_bufs mlp_engine_bufs(struct gpu_state *g, float *loss) {
+    struct mlp_bufs B = {g->d_grad, loss, g->d_gb1e, g->d_wpart, g->d_tpart, g->d_err, MLP_GRID_MAX};
+    return B;
 }
 
 extern "C" int engine_mlp_grad(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
@@ -536,7 +623,11 @@ extern "C" int engine_mlp_grad(struct engine *e, const float *r, const int *pair
     }
     free(tiles);
     hipLaunchKernelGGL(k_mlp_fill, dim3(1), dim3(64), 0, g->stream, g->d_loss, 1LL, 0.0f);
-    if (mlp_gradient(g, g->d_tiles, (int)(n / 16), (float)wE, (float)wF, g->d_loss) < 0) return engine_err_gpu;
+    {
+        const struct mlp_bufs B = mlp_engine_bufs(g, g->d_loss);
+        if (mlp_gradient(g, &B, g->d_sr, g->d_sp, g->d_sE, g->d_sF, g->d_tiles, (int)(n / 16), (float)wE, (float)wF) < 0)
+            return engine_err_gpu;
+    }
     HIPCHECK(hipStreamSynchronize(g->stream));
     HIPCHECK(hipMemcpy(grad, g->d_grad, sizeof(float) * g->mlp_nparams, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(&lossf, g->d_loss, sizeof(float), hipMemcpyDeviceToHost));
@@ -655,9 +746,12 @@ extern "C" int engine_mlp_train(struct engine *e, const float *r, const int *pai
         lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + cos(3.141592653589793 * step / steps));
         c1 = 1.0 / (1.0 - pow(b1, (double)g->mlp_step));
         c2 = 1.0 / (1.0 - pow(b2, (double)g->mlp_step));
-        if (mlp_gradient(g, g->d_tiles + pos, nb, (float)wE, (float)wF, g->d_loss + step) < 0) {
-            free(tiles);
-            return engine_err_gpu;
+        {
+            const struct mlp_bufs B = mlp_engine_bufs(g, g->d_loss + step);
+            if (mlp_gradient(g, &B, g->d_sr, g->d_sp, g->d_sE, g->d_sF, g->d_tiles + pos, nb, (float)wE, (float)wF) < 0) {
+                free(tiles);
+                return engine_err_gpu;
+            }
         }
         hipLaunchKernelGGL(k_mlp_adam, dim3((g->mlp_nparams + 255) / 256), dim3(256), 0, g->stream, g->d_theta,
                            g->d_grad, g->d_adam_m, g->d_adam_v, g->mlp_nparams, (float)lr, b1, b2, (float)c1,

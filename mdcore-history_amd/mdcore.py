@@ -132,6 +132,7 @@ def library():
         "potential_create_Coulomb": (C.c_void_p, [C.c_double] * 4),
         "potential_create_harmonic": (C.c_void_p, [C.c_double] * 5),
         "potential_create_LJ126_RF": (C.c_void_p, [C.c_double] * 8),
+        "potential_create_LJ126_switch_RF": (C.c_void_p, [C.c_double] * 9),
         "potential_getdata": (C.c_int, [C.c_void_p, _ip, _dp, _dp, _dp, C.c_int]),
         "potential_eval": (None, [C.c_void_p, C.c_double, _dp, _dp]),
         "potential_free": (None, [C.c_void_p]),
@@ -241,6 +242,11 @@ def harmonic(a, b, K, r0, tol):
 def LJ126_RF(a, b, A, B, q, ke, eps_rf, tol):
     return Potential(library().potential_create_LJ126_RF(a, b, A, B, q, ke, eps_rf, tol),
                      f"LJ126_RF({a},{b},{A},{B},{q},{ke},{eps_rf},{tol})")
+
+
+def LJ126_switch_RF(a, b, A, B, s, q, ke, eps_rf, tol):
+    return Potential(library().potential_create_LJ126_switch_RF(a, b, A, B, s, q, ke, eps_rf, tol),
+                     f"LJ126_switch_RF({a},{b},{A},{B},{s},{q},{ke},{eps_rf},{tol})")
 
 
 # ---------------------------------------------------------------- engine

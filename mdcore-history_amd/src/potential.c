@@ -749,14 +749,41 @@ static double rf_f6p(double r) {
     return potential_LJ126_6p(r, pc_A, pc_B) + 720.0 * pc_ke * pc_q / r7;
 }
 
+static double rfs_f(double r) {
+    return potential_LJ126(r, pc_A, pc_B) * potential_switch(r, pc_s, pc_b) +
+           pc_ke * pc_q * (1.0 / r + pc_krf * r * r - pc_crf);
+}
+static double rfs_fp(double r) {
+    return potential_LJ126_p(r, pc_A, pc_B) * potential_switch(r, pc_s, pc_b) +
+           potential_LJ126(r, pc_A, pc_B) * potential_switch_p(r, pc_s, pc_b) +
+           pc_ke * pc_q * (-1.0 / (r * r) + 2.0 * pc_krf * r);
+}
+
+static void rf_constants(double b, double eps_rf) {
+    pc_krf = (eps_rf <= 0.0) ? 1.0 / (2.0 * b * b * b) : (eps_rf - 1.0) / ((2.0 * eps_rf + 1.0) * b * b * b);
+    pc_crf = 1.0 / b + pc_krf * b * b;
+}
+
+struct potential *potential_create_LJ126_switch_RF(double a, double b, double A, double B, double s, double q,
+                                                   double ke, double eps_rf, double tol) {
+    pc_A = A;
+    pc_B = B;
+    pc_s = s;
+    pc_b = b;
+    pc_q = q;
+    pc_ke = ke;
+    rf_constants(b, eps_rf);
+    return build(rfs_f, rfs_fp, rf_f6p, a, b, tol,
+                 ((A != 0.0 || B != 0.0) ? potential_flag_LJ126 : 0) | potential_flag_Coulomb);
+}
+
 struct potential *potential_create_LJ126_RF(double a, double b, double A, double B, double q, double ke,
                                             double eps_rf, double tol) {
     pc_A = A;
     pc_B = B;
     pc_q = q;
     pc_ke = ke;
-    pc_krf = (eps_rf <= 0.0) ? 1.0 / (2.0 * b * b * b) : (eps_rf - 1.0) / ((2.0 * eps_rf + 1.0) * b * b * b);
-    pc_crf = 1.0 / b + pc_krf * b * b;
+    rf_constants(b, eps_rf);
     return build(rf_f, rf_fp, rf_f6p, a, b, tol,
                  ((A != 0.0 || B != 0.0) ? potential_flag_LJ126 : 0) | potential_flag_Coulomb);
 }

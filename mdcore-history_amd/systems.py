@@ -102,21 +102,25 @@ def mixture_species(nspecies=20, seed=20):
 
 
 def mixture(N=1048576, density=100.0, seed=7, rmin=0.15, cutoff=1.2, cell=0.6, nspecies=20, a=0.12, tol=1e-3,
-            eps_rf=0.0):
+            eps_rf=0.0, switch=None):
+    """switch: None for plain LJ + reaction field tables; a distance s < cutoff
+    switches the LJ part off between s and the cutoff (force-continuous)."""
     box = (N / density) ** (1.0 / 3.0)
     sig, eps, q, mass = mixture_species(nspecies, seed)
     types = md.gen_types(N, nspecies, 1, seed + 10)
     counts = np.bincount(types - 1, minlength=nspecies)
     q = q - np.dot(counts, q) / N  # count-weighted shift: sum_i q_i = 0
     x = md.gen_rsa(N, [box] * 3, rmin, seed + 20)
-    pots = mixture_pots(sig, eps, q, cutoff, a, eps_rf, tol)
+    pots = mixture_pots(sig, eps, q, cutoff, a, eps_rf, tol, switch)
     return System("mixture", np.zeros(3), np.full(3, box), np.full(3, cell), cutoff, x, types,
                   np.concatenate([[0.0], mass]), pots, None, None,
-                  {"box": box, "sigma": sig, "eps": eps, "q": q, "density": density, "a": a, "eps_rf": eps_rf})
+                  {"box": box, "sigma": sig, "eps": eps, "q": q, "density": density, "a": a, "eps_rf": eps_rf,
+                   "switch": switch})
 
 
-def mixture_pots(sig, eps, q, cutoff, a, eps_rf, tol):
-    """LJ 12-6 (Lorentz-Berthelot) + reaction-field tables of every species pair."""
+def mixture_pots(sig, eps, q, cutoff, a, eps_rf, tol, switch=None):
+    """LJ 12-6 (Lorentz-Berthelot) + reaction-field tables of every species
+    pair; with switch = s the LJ part is switched off from s to the cutoff."""
     pots = []
     n = len(sig)
     for s in range(n):
@@ -125,14 +129,18 @@ def mixture_pots(sig, eps, q, cutoff, a, eps_rf, tol):
             ep = math.sqrt(eps[s] * eps[t])
             A = 4 * ep * sg ** 12
             B = 4 * ep * sg ** 6
-            pots.append((s + 1, t + 1, md.LJ126_RF(a, cutoff, A, B, q[s] * q[t], KE, eps_rf, tol)))
+            if switch is None:
+                pots.append((s + 1, t + 1, md.LJ126_RF(a, cutoff, A, B, q[s] * q[t], KE, eps_rf, tol)))
+            else:
+                pots.append((s + 1, t + 1, md.LJ126_switch_RF(a, cutoff, A, B, switch, q[s] * q[t], KE, eps_rf,
+                                                              tol)))
     return pots
 
 
 def with_tolerance(sys, tol):
     """The same mixture with its tables rebuilt at another tolerance."""
     m = sys.meta
-    pots = mixture_pots(m["sigma"], m["eps"], m["q"], sys.cutoff, m["a"], m["eps_rf"], tol)
+    pots = mixture_pots(m["sigma"], m["eps"], m["q"], sys.cutoff, m["a"], m["eps_rf"], tol, m.get("switch"))
     return System(sys.name, sys.origin, sys.dim, sys.L, sys.cutoff, sys.x, sys.types, sys.masses, pots, sys.excl,
                   sys.v, dict(m, tol=tol))
 
@@ -176,3 +184,34 @@ def tables(sys):
         pind[tj * mt + ti] = k
     return {"max_type": mt, "pind": pind, "n": np.array(ns, np.int32), "alpha": np.concatenate(alphas),
             "c": np.concatenate(cs), "off": np.array(offs, np.int32)}
+
+
+# Stages (dt in ps, steps, steps between rescales) that take the 1M mixture's
+# random-addition start, whose closest pairs sit deep in each other's
+# repulsion, to a state that integrates at 1 fs.
+MIXTURE_EQUILIBRATION = ((2e-5, 100, 10), (1e-4, 200, 20), (5e-4, 300, 50), (1e-3, 400, 100))
+
+
+def equilibrate(eng, sysm, T=300.0, stages=MIXTURE_EQUILIBRATION, seed=17, log=None):
+    """Device-resident steps from Maxwell velocities at T, the velocities
+    rescaled to T after every `every` steps (the reference's drivers rescale
+    p->v toward the target temperature, examples/argon/argon.c:286-299),
+    with the time step raised stage by stage.  Returns the temperature and
+    potential energy at the end of each stage; leaves the engine's dt at the
+    last stage's."""
+    dof = 3 * sysm.N - 3
+    eng.set_velocities(md.gen_maxwell(sysm.masses[sysm.types], KB * T, seed))
+    out = []
+    for dt, nsteps, every in stages:
+        eng.setdt(dt)
+        for _ in range(max(1, nsteps // every)):
+            ep, ek = eng.gpu_steps(every)
+            Tcur = 2.0 * ek[-1] / (dof * KB)
+            if not (np.isfinite(Tcur) and np.isfinite(ep[-1])):
+                raise RuntimeError(f"equilibration diverged at dt {dt}: T {Tcur}, epot {ep[-1]}")
+            if Tcur > 0.0:
+                eng.set_velocities(eng.velocities() * math.sqrt(T / Tcur))
+            if log:
+                log(dt, Tcur, float(ep[-1]))
+        out.append((dt, Tcur, float(ep[-1])))
+    return out

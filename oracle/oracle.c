@@ -19,6 +19,7 @@
  *   table evaluation                     potential_eval.h:67-104
  *   exclusions                           exclusion.c:81-282 (scalar branch)
  *   energy sum over cells                engine.c:1963-1985
+ *   kick-drift (or_advance)              engine.c:1945-2052, threads over cells
  * The scalar virial sum(dx . F_i) = -sum(f r^2) and the projected window
  * (OR_WINDOW_PROJ) have no reference counterpart.
  */
@@ -77,6 +78,10 @@ struct or_system {
     int keep_sort;
     /* pairs evaluated by each task in the last or_nonbond */
     long long *tcount;
+    /* or_advance: velocities by id, inverse mass by type, and the cell
+       offsets of the last or_nonbond's force buffer */
+    double *v, *imass;
+    int *foff;
     /* per-thread scratch of or_nonbond, kept between calls */
     int nbuf;
     double *fbuf, *ebuf;
@@ -488,6 +493,9 @@ void or_free(or_system *s) {
     free(s->tj);
     free(s->tsid);
     free(s->tcount);
+    free(s->v);
+    free(s->imass);
+    free(s->foff);
     free(s->pind);
     free(s->tn);
     free(s->toff);
@@ -766,8 +774,72 @@ int or_nonbond(or_system *s, int mode, int nthreads, double *f, double *energies
         counts[0] = np;
         counts[1] = nx;
     }
-    free(foff);
+    free(s->foff);
+    s->foff = foff;
     return 0;
+}
+
+/* ------------------------------------------------------------ advance */
+
+int or_set_motion(or_system *s, const double *v, const double *imass_by_type) {
+    if (s->v == NULL) s->v = (double *)calloc((size_t)3 * s->N + 1, sizeof(double));
+    if (s->imass == NULL) s->imass = (double *)calloc(s->max_type + 1, sizeof(double));
+    if (s->v == NULL || s->imass == NULL) return -1;
+    if (v) memcpy(s->v, v, sizeof(double) * 3 * s->N);
+    if (imass_by_type) memcpy(s->imass, imass_by_type, sizeof(double) * s->max_type);
+    return 0;
+}
+
+typedef struct {
+    or_system *s;
+    double dt;
+    int c0, c1, out;
+} or_adv_job;
+
+static void *advance_main(void *arg) {
+    or_adv_job *J = (or_adv_job *)arg;
+    or_system *s = J->s;
+    const double dt = J->dt;
+    for (int c = J->c0; c < J->c1; c++) {
+        double *x = s->cx[c];
+        const double *f = &s->fbuf[3 * (size_t)s->foff[c]];
+        for (int k = 0; k < s->cnt[c]; k++) {
+            double *v = &s->v[3 * (size_t)s->cid[c][k]];
+            const double w = dt * s->imass[s->cty[c][k]];
+            int out = 0;
+            for (int m = 0; m < 3; m++) {
+                v[m] += f[3 * k + m] * w;
+                x[3 * k + m] += dt * v[m];
+                out |= (x[3 * k + m] >= s->h[m]) | (x[3 * k + m] < 0.0);
+            }
+            J->out += out;
+        }
+    }
+    return NULL;
+}
+
+int or_advance(or_system *s, double dt, int nthreads) {
+    pthread_t th[256];
+    or_adv_job J[256];
+    int t, out = 0;
+    if (s->foff == NULL || s->v == NULL || s->imass == NULL) return -1;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    for (t = 0; t < nthreads; t++) {
+        J[t].s = s;
+        J[t].dt = dt;
+        J[t].c0 = (int)((long long)s->ncells * t / nthreads);
+        J[t].c1 = (int)((long long)s->ncells * (t + 1) / nthreads);
+        J[t].out = 0;
+    }
+    if (nthreads == 1)
+        advance_main(&J[0]);
+    else {
+        for (t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, advance_main, &J[t]);
+        for (t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    }
+    for (t = 0; t < nthreads; t++) out += J[t].out;
+    return out;
 }
 
 /* ------------------------------------------------------------ brute force */

@@ -58,6 +58,15 @@ int or_bruteforce(or_system *s, double *f, double *energies, long long *count);
    last sort instead of sorting (runner.c:773, runner_dopair.c:428). */
 int or_set_verlet(or_system *s, const double *xrel, const double *cell_maxdx, int keep_sort);
 
+/* Kick-drift of engine_advance (engine.c:1945-2052) with the forces of the
+   last or_nonbond, over cells on nthreads threads: v += dt f / m, x += dt v.
+   Velocities by id and inverse masses by type come from or_set_motion (v
+   NULL keeps them, zero at first).  Particles are not moved between cells:
+   or_advance returns how many now lie outside their cell (none at dt = 0,
+   the setting of the CPU baseline timings), or -1 before any or_nonbond. */
+int or_set_motion(or_system *s, const double *v, const double *imass_by_type);
+int or_advance(or_system *s, double dt, int nthreads);
+
 /* Introspection. */
 int or_ncells(const or_system *s);
 int or_cell(const or_system *s, int cid, int *count, int *ids, unsigned int *sortflags);

@@ -59,6 +59,8 @@ def lib():
             "or_task_counts": (C.c_int, [C.c_void_p, _llp]),
             "or_cell_x": (C.c_int, [C.c_void_p, C.c_int, _dp]),
             "or_sortlists": (C.c_longlong, [C.c_void_p, _up]),
+            "or_set_motion": (C.c_int, [C.c_void_p, _dp, _dp]),
+            "or_advance": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
             "or_set_verlet": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
             "or_sort_desc": (None, [_up, C.c_int]),
             "or_key": (C.c_uint, [_dp, _dp, C.c_double, C.c_double]),
@@ -223,6 +225,21 @@ class Oracle:
 
     def dscale(self):
         return lib().or_dscale(self.h)
+
+    def set_motion(self, v, imass_by_type):
+        """Velocities by id (None: zero) and inverse masses by type for advance()."""
+        vv = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+        im = np.ascontiguousarray(imass_by_type, np.float64)
+        if lib().or_set_motion(self.h, _p(vv), _p(im)) < 0:
+            raise RuntimeError("or_set_motion failed")
+
+    def advance(self, dt, nthreads=1):
+        """Kick-drift with the last nonbond()'s forces; returns how many
+        particles left their cell (they are not migrated)."""
+        out = lib().or_advance(self.h, float(dt), nthreads)
+        if out < 0:
+            raise RuntimeError("or_advance needs set_motion() and a nonbond() first")
+        return out
 
     def task_counts(self):
         """Pairs evaluated by each task (tasks() order) in the last nonbond()."""

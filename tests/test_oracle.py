@@ -491,3 +491,29 @@ def test_bulk_sortlists_match_per_cell_lists():
     assert len(eng.sortlists()) == 13 * sysm.N
     eng.close()
     o.close()
+
+
+def test_advance_is_the_kick_drift():
+    """or_advance: v += dt f/m, x += dt v per particle with the last pass's
+    forces (engine.c:1945-2052), the same on 1 and 3 threads; at dt = 0
+    nothing moves."""
+    sysm = systems.mixture(3000, seed=2)
+    out = []
+    for threads in (1, 3):
+        o = po.from_system(sysm)
+        r = o.nonbond(po.OR_WINDOW_REF)
+        v0 = np.random.default_rng(1).normal(size=(sysm.N, 3))
+        im = 1.0 / np.where(sysm.masses > 0, sysm.masses, 1.0)
+        o.set_motion(v0, im)
+        x0 = np.concatenate([o.cell_x(c, len(o.cell(c)[0])) for c in range(o.ncells)])
+        ids = np.concatenate([o.cell(c)[0] for c in range(o.ncells)])
+        assert o.advance(0.0, threads) == 0
+        assert np.array_equal(np.concatenate([o.cell_x(c, len(o.cell(c)[0])) for c in range(o.ncells)]), x0)
+        dt = 1e-5
+        o.advance(dt, threads)
+        x1 = np.concatenate([o.cell_x(c, len(o.cell(c)[0])) for c in range(o.ncells)])
+        v1 = v0[ids] + r["f"][ids] * (dt * im[sysm.types[ids]])[:, None]
+        assert np.allclose(x1, x0 + dt * v1, rtol=0, atol=1e-15)
+        out.append(x1)
+        o.close()
+    assert np.array_equal(out[0], out[1])
