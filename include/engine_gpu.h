@@ -123,6 +123,12 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
    and sums, folded per atom in group order.  engine_flag_nosplit keeps one
    item per cell. */
 #define engine_flag_nosplit 1048576
+/* Paired visits (no reference counterpart): the full-shell table kernel
+   visits the neighbour cells at opposite offsets +d and -d of an owner cell
+   together, so that each lane walks its atom's window into both (long on one
+   side where it is short on the other).  engine_flag_unpaired visits them
+   one at a time. */
+#define engine_flag_unpaired 2097152
 
 /* Species pairs: ti[p], tj[p] (ti <= tj) for p < returned count. */
 int engine_mlp_pairs(struct engine *e, int *ti, int *tj, int max);

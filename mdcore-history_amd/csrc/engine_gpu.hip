@@ -686,7 +686,7 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
                 adds its own contiguous run of results in order, so the
                 result stays deterministic.  Excluded pairs take the same
                 route, so a pair and its exclusion cancel exactly. */
-template <bool MLP, bool HALF>
+template <bool MLP, bool HALF, bool PAIRED = false>
 __device__ __forceinline__ void nonbond_body(
     unsigned char *smem, struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
     const int *__restrict__ nbr_off, const float4 *__restrict__ nbs, const double *__restrict__ origin,
@@ -780,65 +780,93 @@ __device__ __forceinline__ void nonbond_body(
                     own_key[sid * 64 + lane] = keyof[(size_t)sid * P.nparts + offc + i0 + lane];
             wave_lds_sync();
 
-            /* Prefetch the first neighbour: unsorted positions + sorted words. */
+            /* Visits: one neighbour entry or, in the paired kernels, the two
+               entries of opposite offsets +d and -d, which share the sort
+               direction and so the owner's keys.  The first of the two is the
+               one whose pair task has the owner as its i cell (bit 6 marks
+               it, bit 7 its partner).  The owner atoms' windows into the two
+               cells are long and short in turn, so that each lane's walk
+               through both is about as long as any other's.  Segment A of
+               the staged records is the first entry's cell, segment B the
+               second's. */
+            int kbeg = 0;
+            if constexpr (PAIRED)
+                if (nn > 0 && (nbr[k0].y & 128)) kbeg = 1; /* opened by its partner in the previous group */
+
+            /* Prefetch a visit's first 64 staged positions: lane t < nA holds
+               atom t of A (unsorted position) and A's window word t, lane
+               nA + u the same for atom u of B. */
             float4 pfx = make_float4(0.f, 0.f, 0.f, 0.f);
             unsigned int pfw = 0;
-            if (nn > 0) {
-                const int4 en = nbr[k0];
-                const int2 cj = cnt_off[en.x];
-                if (lane < cj.x) {
-                    pfx = xf[cj.y + lane];
-                    if (!(en.y & 32)) {
-                        const int sid = en.y & 15, orient = (en.y >> 4) & 1;
-                        pfw = sortlist[13 * (size_t)cj.y + (size_t)sid * cj.x + (orient ? lane : cj.x - 1 - lane)];
+            auto prefetch = [&](int kv) {
+                const int4 ea = nbr[k0 + kv];
+                const int2 ca = cnt_off[ea.x];
+                bool sB = false;
+                int2 cs = ca;
+                int4 es = ea;
+                if constexpr (PAIRED) {
+                    if (ea.y & 64) {
+                        const int4 eb = nbr[k0 + kv + 1];
+                        const int2 cb = cnt_off[eb.x];
+                        sB = lane >= ca.x;
+                        cs = sB ? cb : ca;
+                        es = sB ? eb : ea;
                     }
                 }
-            }
+                const int u = sB ? lane - ca.x : lane;
+                if (u < cs.x) {
+                    pfx = xf[cs.y + u];
+                    if (!(es.y & 32)) {
+                        const int sid = es.y & 15, orient = (es.y >> 4) & 1;
+                        pfw = sortlist[13 * (size_t)cs.y + (size_t)sid * cs.x + (orient ? u : cs.x - 1 - u)];
+                    }
+                }
+            };
+            if (kbeg < nn) prefetch(kbeg);
 
             float hx = 0.f, hy = 0.f, hz = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
             double epot = 0.0, vir = 0.0, eexcl = 0.0;
             unsigned int np = 0, nref = 0, nexcl = 0;
 
-            for (int k = 0; k < nn; k++) {
+            for (int k = kbeg, step = 1; k < nn; k += step) {
                 const int4 en = nbr[k0 + k];
+                const bool paired = PAIRED && (en.y & 64);
+                step = paired ? 2 : 1;
+                const int4 enb = paired ? nbr[k0 + k + 1] : en;
                 const int2 cjc = cnt_off[en.x];
-                const int cj = en.x, nj = cjc.x, offj = cjc.y;
+                const int cj = en.x, nA = cjc.x, offj = cjc.y;
+                int nB = 0, offB = 0;
+                if (paired) {
+                    const int2 cb = cnt_off[enb.x];
+                    nB = cb.x;
+                    offB = cb.y;
+                }
+                const int nj = nA + nB;
                 const int sid = en.y & 15, orient = (en.y >> 4) & 1, self = (en.y >> 5) & 1;
 
-                /* Stage the neighbour, in the order its window is walked:
-                   ascending keys when the owner is the task's i cell,
+                /* Stage the neighbours, each in the order its window is
+                   walked: ascending keys when the owner is the task's i cell,
                    descending otherwise, storage order for the self task.
                    Record = (x, y, z, key' | index << 16 | type << tshift) with
                    key' = key, or 0xffff - key when the owner is the j cell. */
                 wave_lds_sync();
                 if (lane < nj) raw[lane] = pfx;
                 wave_lds_sync();
-                if (lane < nj) {
-                    if (self)
-                        stage[lane] = make_float4(pfx.x, pfx.y, pfx.z,
-                                                  __uint_as_float(((unsigned int)lane << 16) |
-                                                                  ((unsigned int)__float_as_int(pfx.w) << P.tshift)));
-                    else {
-                        const int idx = (int)(pfw >> 16) < nj ? (int)(pfw >> 16) : nj - 1;
-                        const float4 p = idx < 64 ? raw[idx] : xf[offj + idx];
-                        const unsigned int kk = orient ? 0xffffu - (pfw & 0xffffu) : (pfw & 0xffffu);
-                        stage[lane] = make_float4(p.x, p.y, p.z,
-                                                  __uint_as_float(kk | ((unsigned int)idx << 16) |
-                                                                  ((unsigned int)__float_as_int(p.w) << P.tshift)));
-                    }
-                }
-                for (int t = lane + 64; t < nj; t += 64) {
+                for (int t = lane; t < nj; t += 64) {
+                    const bool sB = t >= nA;
+                    const int u = sB ? t - nA : t, ns = sB ? nB : nA, os = sB ? offB : offj, rb = sB ? nA : 0;
+                    const int ori = sB ? 1 : orient;
                     if (self) {
-                        const float4 p = xf[offj + t];
+                        const float4 p = t < 64 ? pfx : xf[offj + t];
                         stage[t] = make_float4(p.x, p.y, p.z,
                                                __uint_as_float(((unsigned int)t << 16) |
                                                                ((unsigned int)__float_as_int(p.w) << P.tshift)));
                     } else {
                         const unsigned int w =
-                            sortlist[13 * (size_t)offj + (size_t)sid * nj + (orient ? t : nj - 1 - t)];
-                        const int idx = (int)(w >> 16) < nj ? (int)(w >> 16) : nj - 1;
-                        const float4 p = xf[offj + idx];
-                        const unsigned int kk = orient ? 0xffffu - (w & 0xffffu) : (w & 0xffffu);
+                            t < 64 ? pfw : sortlist[13 * (size_t)os + (size_t)sid * ns + (ori ? u : ns - 1 - u)];
+                        const int idx = (int)(w >> 16) < ns ? (int)(w >> 16) : ns - 1;
+                        const float4 p = rb + idx < 64 ? raw[rb + idx] : xf[os + idx];
+                        const unsigned int kk = ori ? 0xffffu - (w & 0xffffu) : (w & 0xffffu);
                         stage[t] = make_float4(p.x, p.y, p.z,
                                                __uint_as_float(kk | ((unsigned int)idx << 16) |
                                                                ((unsigned int)__float_as_int(p.w) << P.tshift)));
@@ -846,37 +874,48 @@ __device__ __forceinline__ void nonbond_body(
                 }
                 wave_lds_sync();
 
-                /* Prefetch the next neighbour while this one is processed. */
-                if (k + 1 < nn) {
-                    const int4 en1 = nbr[k0 + k + 1];
-                    const int2 c1 = cnt_off[en1.x];
-                    if (lane < c1.x) {
-                        pfx = xf[c1.y + lane];
-                        if (!(en1.y & 32)) {
-                            const int sid1 = en1.y & 15, or1 = (en1.y >> 4) & 1;
-                            pfw = sortlist[13 * (size_t)c1.y + (size_t)sid1 * c1.x + (or1 ? lane : c1.x - 1 - lane)];
-                        }
-                    }
-                }
+                /* Prefetch the next visit while this one is processed. */
+                if (k + step < nn) prefetch(k + step);
                 if (nj == 0) continue;
 
-                /* Shift from the owner cell to the neighbour (minimum image),
+                /* Shifts from the owner cell to the neighbours (minimum image),
                    rounded to FP32 on the host; the FP64 value is recomputed
                    where needed. */
                 const float4 sfv = nbs[k0 + k];
                 const float sf0 = sfv.x, sf1 = sfv.y, sf2 = sfv.z;
+                float sg0 = 0.f, sg1 = 0.f, sg2 = 0.f;
+                if (paired) {
+                    const float4 sb = nbs[k0 + k + 1];
+                    sg0 = sb.x;
+                    sg1 = sb.y;
+                    sg2 = sb.z;
+                }
 
                 int myk = 0, dns = 0;
                 bool live_ref = true, run = true;
                 if (!self) {
                     dns = P.proj ? en.w : en.z;
-                    const int k0 = (int)(__float_as_uint(stage[0].w) & 0xffffu);
-                    const int kstage0 = orient ? 0xffff - k0 : k0; /* staged keys are complemented when orient */
-                    const int s_imax = orient ? kstage0 : kfl[sid];
-                    const int s_jmin = orient ? kfl[13 + sid] : kstage0;
-                    live_ref = (s_jmin + en.z - s_imax) >= 0;
-                    run = P.proj || live_ref;
+                    if (nA > 0) {
+                        const int kr = (int)(__float_as_uint(stage[0].w) & 0xffffu);
+                        const int kstage0 = orient ? 0xffff - kr : kr; /* staged keys are complemented when orient */
+                        const int s_imax = orient ? kstage0 : kfl[sid];
+                        const int s_jmin = orient ? kfl[13 + sid] : kstage0;
+                        live_ref = (s_jmin + en.z - s_imax) >= 0;
+                        run = P.proj || live_ref;
+                    } else {
+                        live_ref = run = false;
+                    }
                     myk = own_key[sid * 64 + il];
+                }
+                /* Segment B: the owner is the j cell of its task. */
+                int kmaxB = 0;
+                bool runB = false;
+                if (paired && nB > 0) {
+                    const int kr = (int)(__float_as_uint(stage[nA].w) & 0xffffu);
+                    runB = P.proj || (kfl[13 + sid] + enb.z - (0xffff - kr)) >= 0;
+                    const int dmdB = P.maxdx != nullptr ? pair_dmaxdist(P.dscale, P.cutoff, P.maxdx[enb.x], P.maxdx[c])
+                                                        : P.dmaxdist;
+                    kmaxB = dmdB - (myk + (P.proj ? enb.w : enb.z) - 0xffff);
                 }
                 /* Window (runner_dopair.c:455-478): record t is admitted while
                    key_j + dnshift - key_i < dmaxdist (owner is i) or
@@ -1087,12 +1126,112 @@ __device__ __forceinline__ void nonbond_body(
                         }
                     }
                 };
-                if (self)
-                    visit(visit_kind<0>{});
-                else if (orient == 0)
-                    visit(visit_kind<1>{});
-                else
-                    visit(visit_kind<2>{});
+                /* A paired visit: each lane walks its share of its atom's window
+                   in A (the owner is the i cell: energies, virial and counts
+                   are taken there), then, from its first record past that
+                   window, its share of the window in B (forces only). */
+                auto visit_pair = [&]() {
+                    int t = part, seg = run ? 0 : 1;
+                    bool alive = active && (run || runB);
+                    while (wave_any(alive)) {
+                        int h = 0;
+                        for (int it = 0; it < HCAP; it++) {
+                            const int base = seg ? nA : 0, ns = seg ? nB : nA, km = seg ? kmaxB : kmax;
+                            const int tt = base + (t < ns ? t : (ns > 0 ? ns - 1 : 0));
+                            const float4 pj = stage[tt];
+                            const unsigned int wj = __float_as_uint(pj.w);
+                            const bool inwin = alive && t < ns && (int)rec_key(wj) < km;
+                            const bool sw = alive && !inwin && seg == 0;
+                            alive = inwin || (sw && runB);
+                            const float s0 = seg ? sg0 : sf0, s1 = seg ? sg1 : sf1, s2 = seg ? sg2 : sf2;
+                            const float dx = (xi.x - pj.x) - s0, dy = (xi.y - pj.y) - s1, dz = (xi.z - pj.z) - s2;
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            bool hit = inwin && r2 <= P.rc2f;
+                            const bool near = inwin && fabsf(r2 - P.rc2f) <= P.band;
+                            if (wave_any(near)) {
+                                if (near) {
+                                    /* Decide in FP64 exactly as the reference does, in
+                                       the task's own orientation. */
+                                    const int cn = seg ? enb.x : cj, on = seg ? offB : offj;
+                                    const double *xa = &x64[3 * (size_t)a];
+                                    const double *xb = &x64[3 * (size_t)(on + rec_idx(wj, P.imask))];
+                                    const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cn];
+                                    double r2d;
+                                    if (seg == 0)
+                                        r2d = r2_task64(xa, xb, min_image(o_nb[0] - o_own[0], P.dim[0]),
+                                                        min_image(o_nb[1] - o_own[1], P.dim[1]),
+                                                        min_image(o_nb[2] - o_own[2], P.dim[2]));
+                                    else
+                                        r2d = r2_task64(xb, xa, min_image(o_own[0] - o_nb[0], P.dim[0]),
+                                                        min_image(o_own[1] - o_nb[1], P.dim[1]),
+                                                        min_image(o_own[2] - o_nb[2], P.dim[2]));
+                                    hit = r2d <= P.cutoff2;
+                                }
+                            }
+                            hitl[h * 64 + lane] = (unsigned short)tt;
+                            h += hit ? 1 : 0;
+                            if (sw) {
+                                t = part;
+                                seg = 1;
+                            } else if (inwin)
+                                t += NPi;
+                            if (!wave_any(alive)) break;
+                        }
+                        /* Evaluate the kept hits, two per lane per round. */
+                        for (int s = 0; wave_any(s < h); s += 2) {
+                            const bool va = s < h, vb = s + 1 < h;
+                            const int ta = va ? hitl[s * 64 + lane] : 0;
+                            const int tb = vb ? hitl[(s + 1) * 64 + lane] : ta;
+                            const float4 pa = stage[ta], pb = stage[tb];
+                            const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
+                            const float4 ha = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wa, P.tshift)));
+                            const float4 hb = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wb, P.tshift)));
+                            const bool ia = ta < nA, ib = tb < nA;
+                            const float dxa = (xi.x - pa.x) - (ia ? sf0 : sg0), dya = (xi.y - pa.y) - (ia ? sf1 : sg1),
+                                        dza = (xi.z - pa.z) - (ia ? sf2 : sg2);
+                            const float dxb = (xi.x - pb.x) - (ib ? sf0 : sg0), dyb = (xi.y - pb.y) - (ib ? sf1 : sg1),
+                                        dzb = (xi.z - pb.z) - (ib ? sf2 : sg2);
+                            const float r2a = dxa * dxa + dya * dya + dza * dza;
+                            const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+                            const bool ua = va && __float_as_int(ha.w) != -1;
+                            const bool ub = vb && __float_as_int(hb.w) != -1;
+                            float ea, fa, eb, fb;
+                            pot_eval32r2(ha, hb, ua, ub, rs_coef, r2a, r2b, ea, fa, eb, fb);
+                            lfx -= fa * dxa + fb * dxb;
+                            lfy -= fa * dya + fb * dyb;
+                            lfz -= fa * dza + fb * dzb;
+                            if (ua && ia) {
+                                le += ea;
+                                lv -= fa * r2a;
+                                lnp++;
+                                lnref += (!P.proj || (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
+                            }
+                            if (ub && ib) {
+                                le += eb;
+                                lv -= fb * r2b;
+                                lnp++;
+                                lnref += (!P.proj || (live_ref && ((int)rec_key(wb) + en.z - myk) < dmd)) ? 1u : 0u;
+                            }
+                        }
+                    }
+                };
+                if constexpr (PAIRED) {
+                    if (paired)
+                        visit_pair();
+                    else if (self)
+                        visit(visit_kind<0>{});
+                    else if (orient == 0)
+                        visit(visit_kind<1>{});
+                    else
+                        visit(visit_kind<2>{});
+                } else {
+                    if (self)
+                        visit(visit_kind<0>{});
+                    else if (orient == 0)
+                        visit(visit_kind<1>{});
+                    else
+                        visit(visit_kind<2>{});
+                }
                 if constexpr (HALF) {
                     /* this visit's forces on the visited atoms, by slot; later
                        chunks of the owner add to the first one's */
@@ -1320,6 +1459,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const struct mlp_net N = {};
     nonbond_body<false, false>(smem, NB_ARGS, N, nullptr);
+}
+
+/* Table mode with paired visits (neighbour cells at opposite offsets
+   together), under the 4- and the 5-wave register budgets. */
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbond_paired(NB_PARAMS) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const struct mlp_net N = {};
+    nonbond_body<false, false, true>(smem, NB_ARGS, N, nullptr);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_paired_wide(NB_PARAMS) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const struct mlp_net N = {};
+    nonbond_body<false, false, true>(smem, NB_ARGS, N, nullptr);
 }
 
 /* Table mode for systems with many more cells than resident waves: the same
@@ -1572,6 +1725,8 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipFree(g->d_nbr);
     (void)hipFree(g->d_nbs);
     (void)hipFree(g->d_nbr_off);
+    (void)hipFree(g->d_nbr_p);
+    (void)hipFree(g->d_nbs_p);
     (void)hipFree(g->d_nbr_h);
     (void)hipFree(g->d_nbs_h);
     (void)hipFree(g->d_nbr_off_h);
@@ -1831,6 +1986,88 @@ extern "C" int engine_gpu_load(struct engine *e) {
         HIPCHECK(hipMalloc(&g->d_nbs, sizeof(float4) * (total > 0 ? total : 1)));
         HIPCHECK(hipMemcpy(g->d_nbs, nbs, sizeof(float4) * (total > 0 ? total : 1), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(g->d_nbr_off, off, sizeof(int) * (s->nr_cells + 1), hipMemcpyHostToDevice));
+        /* Paired order: per owner the self entry, then each entry whose task
+           has the owner as its i cell followed by the entry at the opposite
+           cell offset with the same sort direction (the owner is its task's
+           j cell), then the entries left over, each list in task order. */
+        {
+            int4 *pn = (int4 *)malloc(sizeof(int4) * (total > 0 ? total : 1));
+            float4 *ps = (float4 *)malloc(sizeof(float4) * (total > 0 ? total : 1));
+            int *used = (int *)calloc(maxnbr + 1, sizeof(int));
+            int *slot = (int *)malloc(sizeof(int) * 17 * 17 * 17);
+            if (pn == NULL || ps == NULL || used == NULL || slot == NULL) {
+                free(pn);
+                free(ps);
+                free(used);
+                free(slot);
+                return gerror_(engine_err_malloc, "malloc failed");
+            }
+            for (k = 0; k < 17 * 17 * 17; k++) slot[k] = -1;
+            auto ofs_key = [&](const float4 v) {
+                int d[3];
+                const float sh[3] = {v.x, v.y, v.z};
+                for (int m = 0; m < 3; m++) {
+                    d[m] = (int)lrint(sh[m] / s->h[m]);
+                    d[m] = d[m] < -8 ? -8 : (d[m] > 8 ? 8 : d[m]);
+                }
+                return ((d[0] + 8) * 17 + (d[1] + 8)) * 17 + (d[2] + 8);
+            };
+            for (cid = 0; cid < s->nr_cells; cid++) {
+                const int b0 = off[cid], n = off[cid + 1] - off[cid];
+                int q = b0;
+                for (k = 0; k < n; k++) used[k] = 0;
+                /* the entries the owner is the j cell of, by cell offset */
+                for (k = 0; k < n; k++)
+                    if (!(nbr[b0 + k].y & 32) && (nbr[b0 + k].y & 16)) {
+                        const int key = ofs_key(nbs[b0 + k]);
+                        if (slot[key] < 0) slot[key] = k;
+                    }
+                for (k = 0; k < n; k++)
+                    if (nbr[b0 + k].y & 32) {
+                        pn[q] = nbr[b0 + k];
+                        ps[q++] = nbs[b0 + k];
+                        used[k] = 1;
+                    }
+                for (k = 0; k < n; k++) {
+                    const int4 ea = nbr[b0 + k];
+                    if (used[k] || (ea.y & 48)) continue;
+                    const float4 sa = nbs[b0 + k];
+                    const int key = ofs_key(make_float4(-sa.x, -sa.y, -sa.z, 0.f));
+                    const int m = slot[key];
+                    used[k] = 1;
+                    pn[q] = ea;
+                    ps[q++] = sa;
+                    if (m >= 0 && !used[m] && (nbr[b0 + m].y & 15) == (ea.y & 15)) {
+                        used[m] = 1;
+                        pn[q - 1].y |= 64;
+                        pn[q] = nbr[b0 + m];
+                        pn[q].y |= 128;
+                        ps[q++] = nbs[b0 + m];
+                    }
+                }
+                for (k = 0; k < n; k++)
+                    if (!used[k]) {
+                        pn[q] = nbr[b0 + k];
+                        ps[q++] = nbs[b0 + k];
+                    }
+                for (k = 0; k < n; k++)
+                    if (!(nbr[b0 + k].y & 32) && (nbr[b0 + k].y & 16)) slot[ofs_key(nbs[b0 + k])] = -1;
+            }
+            free(used);
+            free(slot);
+            if (hipMalloc(&g->d_nbr_p, sizeof(int4) * (total > 0 ? total : 1)) != hipSuccess ||
+                hipMalloc(&g->d_nbs_p, sizeof(float4) * (total > 0 ? total : 1)) != hipSuccess ||
+                hipMemcpy(g->d_nbr_p, pn, sizeof(int4) * (total > 0 ? total : 1), hipMemcpyHostToDevice) !=
+                    hipSuccess ||
+                hipMemcpy(g->d_nbs_p, ps, sizeof(float4) * (total > 0 ? total : 1), hipMemcpyHostToDevice) !=
+                    hipSuccess) {
+                free(pn);
+                free(ps);
+                return gerror_(engine_err_gpu, "neighbour list upload failed");
+            }
+            free(pn);
+            free(ps);
+        }
         free(cnt);
         free(fill);
         free(off);
@@ -2047,7 +2284,7 @@ extern "C" int engine_gpu_load(struct engine *e) {
     HIPCHECK(hipHostMalloc((void **)&g->h_tot, sizeof(double) * 4, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&g->h_tot_cnt, sizeof(unsigned long long) * 4, hipHostMallocDefault));
 
-    for (int k = 0; k < 6; k++) g->lds_nb[k] = 0;
+    for (int k = 0; k < 8; k++) g->lds_nb[k] = 0;
     return engine_err_ok;
 }
 
@@ -2077,6 +2314,7 @@ int gpu_upload(struct engine *e) {
     /* surrogate mode: the network evaluation dominates a visit, so the half
        shell's single evaluation per pair wins at any occupancy */
     g->mlp_half = !(e->flags & engine_flag_fullshell);
+    g->paired = !(e->flags & engine_flag_unpaired);
     g->P.nparts = n;
     g->P.nids = g->nids;
     g->P.maxn = maxc > 0 ? maxc : 1;
@@ -2099,7 +2337,10 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
        the half-shell table kernels (4 / 5 waves), 5 the half-shell
        surrogate kernel */
     const int wide = g->ncells >= 4 * 16 * g->ncu;
-    const int kern = mlp ? (g->mlp_half ? 5 : 1) : g->half ? (wide ? 4 : 3) : (wide ? 2 : 0);
+    const int kern = mlp      ? (g->mlp_half ? 5 : 1)
+                     : g->half ? (wide ? 4 : 3)
+                     : g->paired ? (wide ? 7 : 6)
+                                 : (wide ? 2 : 0);
     struct nb_params P = g->P;
     int nb;
     size_t lds = 0;
@@ -2112,7 +2353,7 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     lds += sizeof(int) * 32;
     lds = (lds + 15) & ~(size_t)15;
     P.lds_stage = (int)lds;
-    lds += sizeof(float4) * maxn;
+    lds += sizeof(float4) * maxn * (kern >= 6 ? 2 : 1); /* paired visits stage two cells */
     P.lds_hit = (int)lds;
     lds += sizeof(unsigned short) * 64 * (mlp ? MD_HCAP_MLP : MD_HCAP);
     lds = (lds + 15) & ~(size_t)15;
@@ -2120,9 +2361,9 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     if (mlp) lds += 16 * 64 * MD_HCAP_MLP + 4 * 64 * MD_XCAP_MLP;
     lds = (lds + 15) & ~(size_t)15;
     P.lds_jf = (int)lds;
-    if (kern >= 3) lds += sizeof(float) * 4 * 3 * maxn;
+    if (kern >= 3 && kern <= 5) lds += sizeof(float) * 4 * 3 * maxn;
     if (lds > 64 * 1024) return gerror_(engine_err_maxparts, "LDS staging exceeds 64 KiB");
-    if (kern >= 3 && (g->d_jpart == NULL || g->jcap < (int)maxn)) {
+    if (kern >= 3 && kern <= 5 && (g->d_jpart == NULL || g->jcap < (int)maxn)) {
         /* per half-shell entry, room for the visited cell's atoms; no
            pending pass may still use the old buffer */
         HIPCHECK(hipStreamSynchronize(g->stream));
@@ -2156,6 +2397,10 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_half, 64, lds));
         else if (kern == 4)
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_half_wide, 64, lds));
+        else if (kern == 6)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_paired, 64, lds));
+        else if (kern == 7)
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond_paired_wide, 64, lds));
         else
             HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonbond, 64, lds));
         if (per_cu < 1) per_cu = 1;
@@ -2219,15 +2464,22 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue, gpu_mlp_net(g), g->d_ptype);
+    else if (kern >= 6)
+        hipLaunchKernelGGL(kern == 7 ? k_nonbond_paired_wide : k_nonbond_paired, dim3(grid), dim3(64), lds, g->stream,
+                           P, g->d_cnt_off, g->d_nbr_p,
+                           g->d_nbr_off, g->d_nbs_p, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
+                           g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
+                           g->d_index_of_id, g->d_cell_of, O, g->d_queue);
     else
         hipLaunchKernelGGL(kern == 2 ? k_nonbond_wide : k_nonbond, dim3(grid), dim3(64), lds, g->stream,
                            P, g->d_cnt_off, g->d_nbr,
                            g->d_nbr_off, g->d_nbs, g->d_origin, g->d_loc, g->d_xf, g->d_x, g->d_id, g->d_sortlist,
                            g->d_keyof, g->d_ptab, (const float4 *)g->d_coeffs, g->d_excl_off, g->d_excl_list,
                            g->d_index_of_id, g->d_cell_of, O, g->d_queue);
-    if (kern >= 3 || g->nsplit > 1)
+    if ((kern >= 3 && kern <= 5) || g->nsplit > 1)
         hipLaunchKernelGGL(k_fold, dim3(g->ncells), dim3(64), 0, g->stream, g->ncells, g->d_cnt_off, g->nsplit,
-                           g->nparts, (const float4 *)g->d_phi, (const float4 *)g->d_plo, kern >= 3 ? 1 : 0,
+                           g->nparts, (const float4 *)g->d_phi, (const float4 *)g->d_plo,
+                           (kern >= 3 && kern <= 5) ? 1 : 0,
                            g->d_self_entry, g->d_inv_off, g->d_inv_list, (const float4 *)g->d_jpart, g->jcap,
                            g->d_fhi, g->d_flo);
     if (timed) HIPCHECK(hipEventRecord(g->ev[2], g->stream));

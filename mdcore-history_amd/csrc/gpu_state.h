@@ -62,15 +62,22 @@ struct gpu_state {
     struct nb_params P;
     /* [0] table mode, [1] surrogate mode, [2] table mode with the 5-wave
        budget, [3] / [4] the half-shell table kernels with 4 / 5 waves, [5]
-       the half-shell surrogate kernel */
-    int grid_nb[6], ncu;
-    size_t lds_nb[6];
+       the half-shell surrogate kernel, [6] / [7] the paired table kernels
+       with 4 / 5 waves */
+    int grid_nb[8], ncu;
+    size_t lds_nb[8];
     /* static */
     float4 *d_ptab;
     float *d_coeffs;
     int4 *d_nbr;
     float4 *d_nbs; /* per neighbour entry: FP32 minimum-image shift owner -> neighbour */
     int *d_nbr_off;
+    /* the same entries with those of opposite offsets side by side: the
+       first of a pair (its owner is the task's i cell) has bit 6 of .y set,
+       its partner bit 7 */
+    int4 *d_nbr_p;
+    float4 *d_nbs_p;
+    int paired; /* set unless engine_flag_unpaired */
     /* half shell: each task once at its i cell; the self entry of each cell;
        the entries that visit each cell as a task's j cell; per entry the
        forces on the visited atoms (jcap slots) */

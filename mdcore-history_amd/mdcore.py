@@ -34,6 +34,7 @@ engine_flag_fullshell = 262144
 engine_flag_halfshell = 524288
 engine_gpu_half_mean = 48
 engine_flag_nosplit = 1048576
+engine_flag_unpaired = 2097152
 engine_flag_unsorted = 512
 
 space_periodic_full = 7

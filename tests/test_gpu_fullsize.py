@@ -67,19 +67,20 @@ def test_full_size_parity(name):
 @pytest.mark.parametrize("name", sorted(FULL))
 def test_full_size_modes_agree(name):
     """At full size, the default pass and the passes with the other pair-kernel
-    shell and without the owner split count the same pairs and exclusions and
-    agree to FP32 summation order."""
+    shell, without the owner split and without paired visits count the same
+    pairs and exclusions and agree to FP32 summation order."""
     sysm = FULL[name]()
     res = {}
     for label, flags in (("default", 0), ("full", md.engine_flag_fullshell), ("half", md.engine_flag_halfshell),
-                         ("nosplit", md.engine_flag_nosplit)):
+                         ("nosplit", md.engine_flag_nosplit),
+                         ("unpaired", md.engine_flag_fullshell | md.engine_flag_unpaired)):
         eng = systems.build_engine(sysm, md.engine_flag_gpu | flags)
         eng.shuffle()
         eng.nonbond()
         res[label] = (eng.forces().copy(), eng.gpu_results())
         eng.close()
     f0, g0 = res["default"]
-    for label in ("full", "half", "nosplit"):
+    for label in ("full", "half", "nosplit", "unpaired"):
         f, g = res[label]
         err = float(np.sqrt(np.sum((f - f0) ** 2) / np.sum(f0 ** 2)))
         print(f"{name} {label}: pairs {g['pairs']}, rel diff to default {err:.2e}", flush=True)

@@ -5,7 +5,8 @@ including pair tasks that the reference window's unsigned break skips
 whole), exclusion counts.  Floating point: per-atom force relative RMSE <= 1e-5 and
 energy / virial relative error <= 1e-6 against the FP64 oracle (FP32 pair
 arithmetic, two-sum accumulation, FP64 reductions).  Every system runs with
-both pair-kernel shells (full: each pair at both cells; half: each pair task
+both pair-kernel shells (full: each pair at both cells, the neighbour cells
+at opposite offsets visited together, or one at a time; half: each pair task
 once, opposite forces folded per atom), with each owner cell's neighbours
 split over several work items (the default for these small systems) and
 with one item per cell.
@@ -83,7 +84,8 @@ def dense_big_cells(L=3.9, density=150.0, seed=3):
                           np.array([0.0, 1.0]), [(1, 1, pot)])
 
 
-SHELLS = {"full": md.engine_flag_fullshell, "half": md.engine_flag_halfshell}
+SHELLS = {"full": md.engine_flag_fullshell, "half": md.engine_flag_halfshell,
+          "fullunpaired": md.engine_flag_fullshell | md.engine_flag_unpaired}
 SPLITS = {"split": 0, "nosplit": md.engine_flag_nosplit}
 MODES = {f"{sh}-{sp}": SHELLS[sh] | SPLITS[sp] for sh in sorted(SHELLS) for sp in sorted(SPLITS)}
 
