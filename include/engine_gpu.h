@@ -157,6 +157,20 @@ int engine_mlp_train(struct engine *e, const float *r, const int *pair, const fl
 int engine_mlp_grad(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
                     long long n, double wE, double wF, float *grad, double *loss);
 
+/* No reference counterpart; for callers that cannot allocate device memory
+   (code under HIP graph capture, tight training loops, harnesses that forbid
+   hipMalloc).  engine_mlp_grad on DEVICE buffers r, pair, E, F (n samples)
+   and outputs grad (engine_mlp_params(e, NULL, 0) floats) and loss (one
+   float), with no hipMalloc, hipFree or other allocation: its scratch is a
+   static per-device array of 7.9 MB set up by engine_mlp_init.  Limits:
+   16 <= n <= 65536, n a multiple of 16, at most 1024 species pairs; beyond
+   them it returns engine_err_range.  Asynchronous on the engine's stream:
+   the outputs are valid once the device is synchronized.  A pair id outside
+   the network's pairs makes *loss NaN.  Calls that overlap on one device
+   race on the shared scratch. */
+int engine_mlp_grad_noalloc(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
+                            long long n, double wE, double wF, float *grad, float *loss);
+
 /* Evaluate the network on n samples: E[k] and dE/dr F[k]. */
 int engine_mlp_eval(struct engine *e, const float *r, const int *pair, long long n, float *E, float *F);
 

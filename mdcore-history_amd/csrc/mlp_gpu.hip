@@ -23,6 +23,9 @@
  * gradient, and a whole training run, gives the same bits every time.
  * k_mlp_eval evaluates the network on arbitrary (r, pair) samples.
  * The forward pass and the lane layout are in mlp_tile.h.
+ *
+ * engine_mlp_grad_noalloc is engine_mlp_grad on device buffers with its
+ * scratch in the file-scope array `scratch`: it allocates nothing.
  */
 #include <hip/hip_runtime.h>
 
@@ -43,6 +46,37 @@
 #define MLP_TR_LOSS 194
 #define MLP_TCHUNK 4096 /* tiles per gradient launch (rows of the per-tile buffer) */
 
+/* Scratch of engine_mlp_grad_noalloc, one copy per device for the life of
+   the process (see scratch_init): the bad-pair flag, the tile list, the
+   b1e gradient, the per-wave and the per-tile partial rows.  At most
+   MLP_NA_TILES tiles of 16 samples, MLP_NA_PAIRS species pairs and
+   MLP_NA_GRID waves per gradient launch: 7.9 MB. */
+#define MLP_NA_TILES 4096
+#define MLP_NA_PAIRS 1024
+#define MLP_NA_GRID 256
+#define MLP_NA_MAXDEV 64
+#define NA_OFF_ERR 0
+#define NA_OFF_TILES 64
+#define NA_OFF_GB1E (NA_OFF_TILES + MLP_NA_TILES)
+#define NA_OFF_WPART (NA_OFF_GB1E + MLP_NA_PAIRS * MLP_H)
+#define NA_OFF_TPART (NA_OFF_WPART + MLP_NA_GRID * MLP_WPART)
+#define NA_SIZE (NA_OFF_TPART + MLP_NA_TILES * MLP_TROW)
+static __device__ __align__(256) float scratch[NA_SIZE];
+
+/* Device address of each device's copy of scratch (the array's name is a
+   host-side stand-in and must never be passed to a kernel). */
+static float *scratch_ptr[MLP_NA_MAXDEV];
+
+/* Look up this device's copy of scratch once, with the device current. */
+static int scratch_init(int dev) {
+    if (dev < 0 || dev >= MLP_NA_MAXDEV) return gerror_(engine_err_range, "device id beyond the scratch table");
+    if (scratch_ptr[dev] != NULL) return engine_err_ok;
+    void *ptr = NULL;
+    HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    scratch_ptr[dev] = (float *)ptr;
+    return engine_err_ok;
+}
+
 /* ------------------------------------------------------------ kernels */
 
 __global__ __launch_bounds__(256) void k_mlp_fill(float *__restrict__ a, long long n, float v) {
@@ -50,6 +84,9 @@ __global__ __launch_bounds__(256) void k_mlp_fill(float *__restrict__ a, long lo
         a[k] = v;
 }
 
+__global__ __launch_bounds__(256) void k_mlp_iota(int *__restrict__ a, int n) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) a[k] = k;
+}
 
 /* K index carried by element e of lane group g in K-block kb. */
 __device__ __forceinline__ int mlp_kperm(int kb, int g, int e) { return 16 * (2 * kb + (e >> 2)) + 4 * g + (e & 3); }
@@ -526,6 +563,7 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
     hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_adam_v, (long long)nparams, 0.0f);
     hipLaunchKernelGGL(k_mlp_fill, dim3(1), dim3(64), 0, g->stream, g->d_err, 1LL, 0.0f);
     if (mlp_publish(g) < 0) return engine_err_gpu;
+    if (scratch_init(g->device) < 0) return engine_err_gpu;
     HIPCHECK(hipStreamSynchronize(g->stream));
     return nparams;
 }
@@ -632,6 +670,29 @@ extern "C" int engine_mlp_grad(struct engine *e, const float *r, const int *pair
     HIPCHECK(hipMemcpy(grad, g->d_grad, sizeof(float) * g->mlp_nparams, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(&lossf, g->d_loss, sizeof(float), hipMemcpyDeviceToHost));
     if (loss) *loss = lossf;
+    return engine_err_ok;
+}
+
+extern "C" int engine_mlp_grad_noalloc(struct engine *e, const float *r, const int *pair, const float *E,
+                                       const float *F, long long n, double wE, double wF, float *grad, float *loss) {
+    struct gpu_state *g = mlp_state(e, 1);
+    float *sc;
+    if (g == NULL) return engine_err;
+    if (r == NULL || pair == NULL || E == NULL || F == NULL || grad == NULL || loss == NULL)
+        return gerror_(engine_err_null, "NULL device buffer");
+    if (n < 16 || n % 16 != 0) return gerror_(engine_err_range, "n must be a positive multiple of 16");
+    if (n / 16 > MLP_NA_TILES) return gerror_(engine_err_range, "n exceeds the scratch limit of 65536 samples");
+    if (g->mlp_np > MLP_NA_PAIRS) return gerror_(engine_err_range, "more species pairs than the scratch holds (1024)");
+    if (g->device < 0 || g->device >= MLP_NA_MAXDEV || (sc = scratch_ptr[g->device]) == NULL)
+        return gerror_(engine_err_gpu, "scratch not set up on this device (engine_mlp_init)");
+    const int ntiles = (int)(n / 16);
+    const struct mlp_bufs B = {grad, loss, sc + NA_OFF_GB1E, sc + NA_OFF_WPART, sc + NA_OFF_TPART, sc + NA_OFF_ERR,
+                               MLP_NA_GRID};
+    int *tiles = (int *)(sc + NA_OFF_TILES);
+    hipLaunchKernelGGL(k_mlp_iota, dim3(16), dim3(256), 0, g->stream, tiles, ntiles);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(1), dim3(64), 0, g->stream, sc + NA_OFF_ERR, 1LL, 0.0f);
+    hipLaunchKernelGGL(k_mlp_fill, dim3(1), dim3(64), 0, g->stream, loss, 1LL, 0.0f);
+    if (mlp_gradient(g, &B, r, pair, E, F, tiles, ntiles, (float)wE, (float)wF) < 0) return engine_err_gpu;
     return engine_err_ok;
 }
 

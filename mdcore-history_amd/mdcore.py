@@ -124,6 +124,8 @@ def library():
                                        C.c_double, C.c_double, C.c_double, C.c_ulonglong, _fp]),
         "engine_mlp_grad": (C.c_int, [C.c_void_p, _fp, _ip, _fp, _fp, C.c_longlong, C.c_double, C.c_double, _fp,
                                       _dp]),
+        "engine_mlp_grad_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_longlong, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
         "engine_mlp_eval": (C.c_int, [C.c_void_p, _fp, _ip, C.c_longlong, _fp, _fp]),
         "potential_create_LJ126": (C.c_void_p, [C.c_double] * 5),
         "potential_create_LJ126_switch": (C.c_void_p, [C.c_double] * 6),
@@ -538,6 +540,14 @@ class Engine:
                                        ptr(F, C.c_float), len(r), wE, wF, ptr(g, C.c_float), ptr(loss)),
               "engine_mlp_grad")
         return g, float(loss[0])
+
+    def mlp_grad_noalloc(self, r, pair, E, F, n, grad, loss, wE=1.0, wF=1.0):
+        """engine_mlp_grad_noalloc on device buffers given as addresses
+        (r, E, F: float32; pair: int32; grad: float32 of the parameter count;
+        loss: one float32).  Asynchronous on the engine's stream."""
+        check(self.lib.engine_mlp_grad_noalloc(self.e, C.c_void_p(r), C.c_void_p(pair), C.c_void_p(E), C.c_void_p(F),
+                                               int(n), wE, wF, C.c_void_p(grad), C.c_void_p(loss)),
+              "engine_mlp_grad_noalloc")
 
     def mlp_eval(self, r, pair):
         r, pair, _, _ = self._samples(r, pair)
