@@ -53,6 +53,15 @@ TRAFFIC_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_traffic.json"
 COUNTERS_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_counters.json"
 
 
+_T0 = time.perf_counter()
+
+
+def note(msg):
+    """One progress line on stderr per finished section (the JSON line is the
+    only thing on stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def build_workload(name):
     import systems
     if name == "mixture_1m":
@@ -160,6 +169,7 @@ def config_section(name, args, threads, eng=None, f_gpu=None, ms=None):
         f_gpu = eng.forces()
     pairs = eng.gpu_results()["pairs"]
     r64, cpu = cpu_baseline_config(sysm, threads, mode)
+    note(f"{name}: FP64 CPU baseline {cpu['nonbond_ms']} ms per pass on {threads} threads")
     out = {"workload": desc, "atoms": int(sysm.N), "pairs": int(pairs), "gpu_pass_ms": round(ms["total_ms"], 4),
            "gpu_pairs_per_s": pairs / (ms["total_ms"] * 1e-3), "cpu": cpu,
            "gpu_over_cpu_nonbond": pairs / (ms["total_ms"] * 1e-3) / cpu["pairs_per_s_nonbond"],
@@ -215,6 +225,7 @@ def main():
     ms = eng.gpu_time(args.steps, args.warmup)
     res = eng.gpu_results()
     pairs = res["pairs"]
+    note(f"{args.workload}: {args.steps} timed passes, {ms['total_ms']:.3f} ms per pass")
     step_s = ms["total_ms"] * 1e-3
     value = pairs / step_s
 
@@ -298,6 +309,7 @@ def main():
         out["configs"] = configs
     if not args.no_mlp:
         out["mlp"] = mlp_section(sysm, eng, args, f_table, f64, value)
+        note("surrogate section done")
         if f64 is not None:
             out["force_rmse_vs_fp64"]["mlp"] = out["mlp"].pop("_rmse_fp64")
         else:
@@ -312,11 +324,14 @@ def main():
             md.check(eng.lib.engine_step(eng.e), "engine_step")
         full_s = (time.perf_counter() - t0) / args.engine_steps
         out["ns_per_day_engine_step"] = DT_PS * 1e-3 * 86400.0 / full_s
+        note("engine_step section done")
     if not args.no_nve:
         out["nve"] = nve_section()
+        note("NVE section done")
     eng.close()
     if args.md_steps > 0:
         out["md_step"] = md_step_section(args.md_steps)
+        note("md_step section done")
     print(json.dumps(out), flush=True)
 
 
@@ -459,6 +474,7 @@ def nve_section(steps=10000, dt=0.002, equil=1000):
     for key, sysm, mlp in (("table", base, False), ("mlp", base, True), ("table_switch", switched, False),
                            ("mlp_switch", switched, True)):
         out[key] = nve_drift(sysm, mlp, equil, steps, dt)
+        note(f"NVE argon {key}: drift {out[key]['drift_rel_per_ns']:.2e}/ns")
     out["water_rigid"] = water_nve()
     return out
 
@@ -503,6 +519,7 @@ def md_step_section(nsteps, host_steps=5, dt=0.002):
     1.13 nm cells (cell option 1.1 nm: 0.13 nm skin), and those cells
     re-sorted every step."""
     out = md_step_run(nsteps, host_steps, dt, 1.0, False)
+    note(f"md_step argon 1M: {out['resident_ms_per_step']} ms per resident step")
     out["sort_reuse"] = md_step_run(nsteps, 0, dt, 1.1, True)
     out["same_cells_no_reuse"] = md_step_run(nsteps, 0, dt, 1.1, False)
     return out
