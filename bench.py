@@ -195,6 +195,11 @@ def main():
                     help="resident MD steps timed in the md_step section (0 skips it)")
     ap.add_argument("--md-only", action="store_true", help="run only the md_step section")
     ap.add_argument("--engine-steps", type=int, default=3, help="full engine_step calls timed for ns/day")
+    ap.add_argument("--mixture-nve", choices=["table", "mlp"], default=None,
+                    help="only run the 1,048,576-atom mixture's NVE (switched LJ + reaction-field tables): device "
+                         "equilibration, then --nve-steps steps with the tables or with a surrogate trained on 2^24 "
+                         "samples of the equilibrated state (its own command: it takes minutes)")
+    ap.add_argument("--nve-steps", type=int, default=10000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times, as the "
                          "known-size reference for the FETCH_SIZE / WRITE_SIZE counter scale")
@@ -209,6 +214,9 @@ def main():
 
     if args.md_only:
         print(json.dumps({"md_step": md_step_section(args.md_steps)}), flush=True)
+        return
+    if args.mixture_nve:
+        print(json.dumps({"mixture_nve": mixture_nve(args.mixture_nve == "mlp", args.nve_steps)}), flush=True)
         return
 
     import mdcore as md
@@ -455,15 +463,19 @@ def matched_table(sysm, f_table, target, args, iters=10):
 
 def nve_section(steps=10000, dt=0.002, equil=1000):
     """Energy drift of the 32,000-atom argon system (fcc start at 100 K) over
-    `steps` device-resident NVE steps after `equil` steps: the configuration's
-    plain LJ 12-6 table (not shifted: pairs crossing the cutoff jump in
-    energy), the force-continuous LJ126_switch table (switch from 0.9 nm),
-    and the surrogate pass trained on each table over the equilibrated
-    configuration.  Total energy at step n: PE(x_n) + (KE(v_{n-1/2}) +
-    KE(v_{n+1/2})) / 2; drift = least-squares slope, per ns, relative to
-    |<E>| and in kJ/mol."""
+    `steps` device-resident NVE steps after `equil` steps (surrogate.nve_drift):
+      table         the configuration's plain LJ 12-6 table (not shifted:
+                    pairs crossing the cutoff jump in energy);
+      mlp_switched  the surrogate trained on that table, its energy switched
+                    off to zero between 0.9 nm and the cutoff;
+      table_switch  the force-continuous LJ126_switch table (switch from
+                    0.9 nm), the same function as mlp_switched's target;
+      mlp           the surrogate trained on the LJ126_switch table, no switch
+                    of its own;
+    and the 100k-atom rigid water."""
     import dataclasses
     import mdcore as md
+    import surrogate as sg
     import systems
     base = systems.argon(20)
     m = base.meta
@@ -471,45 +483,97 @@ def nve_section(steps=10000, dt=0.002, equil=1000):
         base, pots=[(1, 1, md.LJ126_switch(0.275, 1.0, m["A"], m["B"], 0.9, 1e-3))])
     out = {"system": "argon_32k: 32,000 LJ atoms, fcc start at 100 K, rc 1.0 nm", "steps": steps, "dt_ps": dt,
            "equilibration_steps": equil}
-    for key, sysm, mlp in (("table", base, False), ("mlp", base, True), ("table_switch", switched, False),
-                           ("mlp_switch", switched, True)):
-        out[key] = nve_drift(sysm, mlp, equil, steps, dt)
+    for key, sysm, mlp, sw in (("table", base, False, None), ("mlp_switched", base, True, 0.9),
+                               ("table_switch", switched, False, None), ("mlp", switched, True, None)):
+        out[key] = sg.nve_drift(sysm, mlp=mlp, switch=sw, equil=equil, steps=steps, dt=dt)
         note(f"NVE argon {key}: drift {out[key]['drift_rel_per_ns']:.2e}/ns")
     out["water_rigid"] = water_nve()
     return out
 
 
-def nve_drift(sysm, mlp, equil, steps, dt):
+def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 << 24):
+    """NVE of the 1,048,576-atom mixture with force-continuous tables (LJ
+    switched off from `switch` to the 1.2 nm cutoff, conducting-boundary
+    reaction field): device-resident equilibration with velocity rescaling
+    (the reference drivers' scheme, examples/argon/argon.c:286-299) up to
+    dt, then `steps` NVE steps.  With mlp the NVE steps evaluate the pairs
+    with a surrogate trained on `samples` samples (half uniform in r, half
+    the equilibrated configuration's pair distances); its force and energy
+    error against the table pass and the FP64 oracle are taken on the
+    equilibrated configuration first.  Drift: least-squares slope of
+    PE(x_n) + (KE(v_{n-1/2}) + KE(v_{n+1/2}))/2, per ns, relative to |<E>|."""
     import dataclasses
     import numpy as np
     import mdcore as md
     import surrogate as sg
     import systems
     t0 = time.perf_counter()
+    sysm = systems.mixture(1048576, switch=switch)
     eng = systems.build_engine(sysm, md.engine_flag_gpu)
-    eng.setdt(dt)
-    eng.gpu_steps(equil)
+    stages = systems.MIXTURE_EQUILIBRATION + ((dt, 400, 100),)
+    eq = systems.equilibrate(eng, sysm, T=T, stages=stages,
+                             log=lambda d, tc, ep: None)
+    note(f"mixture equilibrated: {eq[-1]}")
+    out = {"system": "mixture_1m: 1,048,576 atoms, 20 species, LJ switched from "
+                     f"{switch} nm + reaction field, rc 1.2 nm, 0.6 nm cells",
+           "equilibration": [{"dt_ps": d, "T_K": tc, "epot": ep} for d, tc, ep in eq], "dt_ps": dt, "steps": steps,
+           "pass": "surrogate" if mlp else "table"}
     if mlp:
-        # the surrogate run starts from the same equilibrated state
         s_eq = dataclasses.replace(sysm, x=eng.positions(), v=eng.velocities())
         eng.close()
         eng = systems.build_engine(s_eq, md.engine_flag_gpu)
         eng.setdt(dt)
+        eng.shuffle()
+        eng.nonbond()
+        f_table = eng.forces()
+        r_table = eng.gpu_results()
+        t1 = time.perf_counter()
         dom = sg.pair_domains(eng, s_eq)
-        S = sg.make_samples(eng, s_eq, 1 << 20, dom[0], dom[1], configs=[s_eq.x], seed=5)
+        S = sg.make_samples(eng, s_eq, samples, dom[0], dom[1], configs=[s_eq.x], seed=5)
         eng.mlp_init(*dom, seed=7)
-        eng.mlp_train(*S, steps=10000, batch_tiles=512, lr0=1e-2, lr1=1e-5, seed=11)
+        hist = eng.mlp_train(*S, steps=20000, batch_tiles=1024, lr0=1e-2, lr1=1e-5, seed=11)
+        note(f"surrogate trained: loss {hist[:20].mean():.3e} -> {hist[-50:].mean():.3e}")
         eng.setflags(md.engine_flag_mlp, 0)
-    ep, ek = eng.gpu_steps(steps)
+        eng.nonbond()
+        f_mlp = eng.forces()
+        r_mlp = eng.gpu_results()
+        if str(ROOT / "oracle") not in sys.path:
+            sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as po
+        orc = po.from_system(s_eq)
+        o = orc.nonbond(po.OR_WINDOW_REF, host_cpus()["threads"])
+        orc.close()
+        out["training"] = {"samples": int(len(S[0])), "steps": 20000, "batch": 16384,
+                           "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
+                           "s": round(time.perf_counter() - t1, 1)}
+        out["force_rel_rmse"] = {"mlp_vs_table": rel_rmse(f_mlp, f_table), "mlp_vs_fp64": rel_rmse(f_mlp, o["f"]),
+                                 "table_vs_fp64": rel_rmse(f_table, o["f"])}
+        out["epot_rel_err"] = {"mlp_vs_fp64": abs(r_mlp["epot"] - o["epot"]) / abs(o["epot"]),
+                               "table_vs_fp64": abs(r_table["epot"] - o["epot"]) / abs(o["epot"])}
+        out["pairs_equal"] = r_mlp["pairs"] == r_table["pairs"] == o["pairs"]
+        note(f"surrogate error: {out['force_rel_rmse']}")
+    else:
+        eng.setdt(dt)
+    eps, eks = [], []
+    t2 = time.perf_counter()
+    for k in range(0, steps, chunk):
+        ep, ek = eng.gpu_steps(min(chunk, steps - k))
+        eps.append(ep)
+        eks.append(ek)
+        note(f"mixture NVE ({out['pass']}): {k + len(ep)} steps, E {ep[-1] + ek[-1]:.6e}")
+    wall = time.perf_counter() - t2
     eng.close()
+    ep, ek = np.concatenate(eps), np.concatenate(eks)
     et = ep[1:] + 0.5 * (ek[:-1] + ek[1:])
     rel, absd = sg.drift_per_ns(et, dt)
     h = len(et) // 2
-    return {"drift_rel_per_ns": rel, "drift_kJ_mol_per_ns": absd,
-            "drift_rel_per_ns_halves": [sg.drift_per_ns(et[:h], dt)[0], sg.drift_per_ns(et[h:], dt)[0]],
-            "e_mean": float(et.mean()), "e_std": float(et.std()),
-            "temperature_K": float(2 * ek.mean() / (3 * sysm.N * systems.KB)),
-            "wall_s": round(time.perf_counter() - t0, 1)}
+    out.update({"drift_rel_per_ns": rel, "drift_kJ_mol_per_ns": absd,
+                "drift_rel_per_ns_halves": [sg.drift_per_ns(et[:h], dt)[0], sg.drift_per_ns(et[h:], dt)[0]],
+                "e_mean": float(et.mean()), "e_std": float(et.std()),
+                "temperature_K": float(2 * ek.mean() / ((3 * sysm.N - 3) * systems.KB)),
+                "ms_per_step": round(wall / steps * 1e3, 3),
+                "ns_per_day": dt * 1e-3 * 86400.0 / (wall / steps), "wall_s": round(time.perf_counter() - t0, 1)})
+    return out
 
 
 def md_step_section(nsteps, host_steps=5, dt=0.002):

@@ -93,10 +93,19 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
 
    A species pair is a distinct potential of the engine; pair ids number
    them in the order engine_gpu_load found them.  Per pair p the network
-   covers r in [r_lo[p], r_hi[p]] and predicts E = escale[p] sinh(y0) and
-   dE/dr = fscale[p] sinh(y1) from inputs (r, 1/r) and an 8-dim learned pair
-   embedding: 64 -> 64 hidden units (SiLU), bf16 weights on MFMA with fp32
-   accumulation for the 64x64 layer, and a 64 -> 2 head per pair.
+   covers r in [r_lo[p], r_hi[p]] with one output y(r), from inputs
+   (r, 1/r) and an 8-dim learned pair embedding: 64 -> 64 hidden units
+   (SiLU), bf16 weights on MFMA with fp32 accumulation for the 64x64 layer,
+   and a 64 -> 1 head per pair.  Its r-derivative yd is carried through
+   both layers alongside (forward mode), and
+     E     = S(r) (escale[p] sinh(y) - shift[p]),
+     dE/dr = S'(r) (escale[p] sinh(y) - shift[p]) + S(r) escale[p] cosh(y) yd:
+   the force is the exact derivative of the energy, so the dynamics conserve
+   it.  S is the reference's switch (potential.c:69-100) from rswitch[p] to
+   r_hi[p] (S = 1 when rswitch[p] >= r_hi[p]): with a switch the pair energy
+   and force go to zero at r_hi.  shift[p] is set after every change of the
+   parameters (engine_mlp_init, _train, _params) so that
+   escale sinh(y(r_hi)) - shift equals the FP64 table's energy at r_hi.
    Parameter vector: fp32, engine_mlp_init returns its length.  With
    engine_flag_mlp set, engine_nonbond_gpu evaluates the pairs with
    r_lo[p] <= r <= cutoff through the network and the others through the
@@ -133,10 +142,11 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
 /* Species pairs: ti[p], tj[p] (ti <= tj) for p < returned count. */
 int engine_mlp_pairs(struct engine *e, int *ti, int *tj, int max);
 
-/* Create the network for every species pair with the given domains and
-   output scales and seeded random weights; returns the parameter count. */
+/* Create the network for every species pair with the given domains, energy
+   scales and switch starts (rswitch[p] > r_lo[p]; >= r_hi[p] for no switch)
+   and seeded random weights; returns the parameter count. */
 int engine_mlp_init(struct engine *e, const double *r_lo, const double *r_hi, const double *escale,
-                    const double *fscale, unsigned long long seed);
+                    const double *rswitch, unsigned long long seed);
 
 /* Copy the fp32 parameters out (get != 0) or in (get == 0); theta == NULL
    returns the parameter count. */
@@ -145,8 +155,13 @@ int engine_mlp_params(struct engine *e, float *theta, int get);
 /* Train on n samples (r, pair id, target E, target dE/dr) on the GPU: steps
    Adam steps over shuffled tiles of 16 same-pair samples, batch_tiles tiles
    per step, learning rate cosine-decayed from lr0 to lr1; loss per sample
-   wE (y0 - asinh(E/escale))^2 + wF (y1 - asinh(F/fscale))^2.  loss_hist[steps] (may be
-   NULL) receives the mean loss of each step's batch. */
+   wE (y - a)^2 + wF (yd - ad)^2 / (ad^2 + 1/c^2) with a = asinh(E/escale),
+   ad = F/sqrt(escale^2 + E^2) (its r-derivative for the target's dE/dr = F)
+   and c = (r_hi - r_lo)/2: the output against a, and its r-derivative
+   against ad, relatively where |ad| is large (the repulsive wall).  The
+   targets are the unswitched potential; the switch is applied exactly at
+   evaluation.  loss_hist[steps] (may be NULL) receives the mean loss of
+   each step's batch. */
 int engine_mlp_train(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
                      long long n, int steps, int batch_tiles, double lr0, double lr1, double wE, double wF,
                      unsigned long long seed, float *loss_hist);
@@ -162,7 +177,7 @@ int engine_mlp_grad(struct engine *e, const float *r, const int *pair, const flo
    hipMalloc).  engine_mlp_grad on DEVICE buffers r, pair, E, F (n samples)
    and outputs grad (engine_mlp_params(e, NULL, 0) floats) and loss (one
    float), with no hipMalloc, hipFree or other allocation: its scratch is a
-   static per-device array of 7.9 MB set up by engine_mlp_init.  Limits:
+   static per-device array of 6.8 MB set up by engine_mlp_init.  Limits:
    16 <= n <= 65536, n a multiple of 16, at most 1024 species pairs; beyond
    them it returns engine_err_range.  Asynchronous on the engine's stream:
    the outputs are valid once the device is synchronized.  A pair id outside
@@ -171,7 +186,8 @@ int engine_mlp_grad(struct engine *e, const float *r, const int *pair, const flo
 int engine_mlp_grad_noalloc(struct engine *e, const float *r, const int *pair, const float *E, const float *F,
                             long long n, double wE, double wF, float *grad, float *loss);
 
-/* Evaluate the network on n samples: E[k] and dE/dr F[k]. */
+/* Evaluate the network on n samples: E[k] and dE/dr F[k], switch and shift
+   included. */
 int engine_mlp_eval(struct engine *e, const float *r, const int *pair, long long n, float *E, float *F);
 
 #ifdef __cplusplus

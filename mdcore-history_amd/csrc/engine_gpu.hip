@@ -643,8 +643,9 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane) {
 }
 
 /* Evaluate the dense pair list [0, T) with the network, 16 pairs per tile;
-   f = (dE/dr)/r and e of pair q go to dn_f[q], dn_e[q].  All 64 lanes must
-   call this together (T is wave-uniform). */
+   f = (dE/dr)/r and e of pair q go to dn_f[q], dn_e[q].  dE/dr is the
+   network's own r-derivative, so the forces conserve the energies reported.
+   All 64 lanes must call this together (T is wave-uniform). */
 __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const float *dn_r2, const int *dn_p,
                                                float *dn_f, float *dn_e, int T, int lane) {
     /* layer 2's operands stay in registers for the whole list: reloading
@@ -657,15 +658,15 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
         const int qv = q < T ? q : T - 1;
         const float r = sqrtf(dn_r2[qv]);
         const int p = dn_p[qv];
-        float y0, y1;
+        float y, yd;
         /* keep the network operands' loads inside the loop: hoisting them
            out keeps ~100 registers live through the whole pair kernel */
         asm volatile("" ::: "memory");
-        mlp_infer16_reg(N, w2r, b2r, r, p, lane, y0, y1);
+        mlp_infer16_reg(N, w2r, b2r, r, p, lane, y, yd);
         if (lane < 16 && q < T) {
-            float e, f;
-            mlp_energy_force(N.sc[p], r, y0, y1, e, f);
-            dn_f[q] = f;
+            float e, dedr;
+            mlp_energy_force(N.sc[p], N.shift[p], r, y, yd, e, dedr);
+            dn_f[q] = dedr / r;
             dn_e[q] = e;
         }
     }

@@ -4,9 +4,12 @@
  * potential_eval.h:67-104, which stay the measure of its error and the fallback here).
  *
  * Training runs on the GPU with this file's own kernels:
- *   k_mlp_grad     forward + backward of tiles of 16 samples of one species
- *                  pair.  Layer 2 and both of its backward products
- *                  (dH1 = W2^T dZ2 and dW2 = dZ2 H1^T) run on
+ *   k_mlp_grad     forward (values and r-derivatives) + backward of tiles
+ *                  of 16 samples of one species pair: the loss holds the
+ *                  derivative output yd, so the backward pass runs through
+ *                  the tangent path as well (second derivatives of SiLU).
+ *                  Layer 2 and its backward products (dH1 = W2^T dZ2,
+ *                  dG1 = W2^T dT2, dW2 = dZ2 H1^T + dT2 G1^T) run on
  *                  v_mfma_f32_16x16x32_bf16 with bf16 operands and fp32
  *                  accumulation; the rest is fp32 VALU.
  *   k_mlp_wsum     fixed-order sum of the per-wave partial rows of the
@@ -36,24 +39,25 @@
 #include "gpu_state.h"
 
 #define MLP_WPART (MLP_H * MLP_H + 3 * MLP_H) /* per-wave partial: W2, W1, b2 */
-#define MLP_GRID_MAX 2048
-/* Per-tile row: [0,128) dW3 of the tile's pair (energy head, then force
-   head), [128,130) db3, [130,194) gradient of the pair's b1e, [194] the
-   tile's loss (times 1/n), [195] padding. */
-#define MLP_TROW 196
-#define MLP_TR_B3 128
-#define MLP_TR_B1E 130
-#define MLP_TR_LOSS 194
+/* Waves of a gradient launch (both entry points, so that they sum the same
+   partial rows in the same order) */
+#define MLP_GRID_MAX 256
+/* Per-tile row: [0,64) dW3 of the tile's pair, [64] db3, [65,129) gradient
+   of the pair's b1e, [129] the tile's loss (times 1/n), [130,132) padding. */
+#define MLP_TROW 132
+#define MLP_TR_B3 64
+#define MLP_TR_B1E 65
+#define MLP_TR_LOSS 129
 #define MLP_TCHUNK 4096 /* tiles per gradient launch (rows of the per-tile buffer) */
 
 /* Scratch of engine_mlp_grad_noalloc, one copy per device for the life of
    the process (see scratch_init): the bad-pair flag, the tile list, the
    b1e gradient, the per-wave and the per-tile partial rows.  At most
    MLP_NA_TILES tiles of 16 samples, MLP_NA_PAIRS species pairs and
-   MLP_NA_GRID waves per gradient launch: 7.9 MB. */
+   MLP_NA_GRID waves per gradient launch: 6.8 MB. */
 #define MLP_NA_TILES 4096
 #define MLP_NA_PAIRS 1024
-#define MLP_NA_GRID 256
+#define MLP_NA_GRID MLP_GRID_MAX
 #define MLP_NA_MAXDEV 64
 #define NA_OFF_ERR 0
 #define NA_OFF_TILES 64
@@ -123,21 +127,31 @@ __device__ __forceinline__ float mlp_sum16(float v) {
     return v;
 }
 
-/* One wave per block; each block walks pairs of tiles (32 samples, the K of
-   the dW2 product).  Samples of tile t are s = 16 t + n, n = 0..15, and all
-   16 share species pair sp[16 t].  Loss per sample:
-   wE (y0 - asinh(E/sE))^2 + wF (y1 - asinh(F/sF))^2; gradients are of the
-   sum times inv_n.  Sums over the shared weights go to this wave's partial
-   row, those of the tile's pair to the tile's row of tpart (index ti of
-   tiles[]).  A pair id outside [0, np) is read as pair 0 and sets *err. */
+/* One wave per block; each block walks tiles of 16 samples s = 16 t + n,
+   n = 0..15, which all share species pair sp[16 t].  Loss per sample:
+   wE (y - a)^2 + wF (yd - ad)^2 / (ad^2 + 1/c^2), a = asinh(E/sE),
+   ad = F/sqrt(sE^2 + E^2) its r-derivative for the target's dE/dr = F,
+   c = (r_hi - r_lo)/2: the derivative's error relative where |ad| is large
+   (the repulsive wall), in units of 1/c where it is small; gradients are of
+   the sum times inv_n.  Backward, per sample:
+     dy, dyd            -> dW3 = dy h2 + dyd g2, db3 = dy,
+     dz2 = w3 (dy silu'(z2) + dyd t2 silu''(z2)),  dt2 = w3 dyd silu'(z2),
+     dh1 = W2^T dz2,  dg1 = W2^T dt2                       (MFMA),
+     dz1 = dh1 silu'(z1) + dg1 t1 silu''(z1),  dt1 = dg1 silu'(z1),
+     dW1 = dz1 x^T + dt1 (dx/dr)^T,  db1e = dz1,
+     dW2 = dz2 h1^T + dt2 g1^T: one MFMA K of 32 per tile, the 16 value
+     columns and the 16 derivative columns (through LDS).
+   Sums over the shared weights go to this wave's partial row, those of the
+   tile's pair to the tile's row of tpart (index ti of tiles[]).  A pair id
+   outside [0, np) is read as pair 0 and sets *err. */
 __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *__restrict__ sr,
                                                  const int *__restrict__ sp, const float *__restrict__ tE,
                                                  const float *__restrict__ tF, const int *__restrict__ tiles,
                                                  int ntiles, float wE, float wF, float inv_n, int np,
                                                  float *__restrict__ tpart, float *__restrict__ partial,
                                                  float *__restrict__ err) {
-    __shared__ __attribute__((aligned(16))) __bf16 h1t[MLP_H][32]; /* [in unit][sample of the tile pair] */
-    __shared__ __attribute__((aligned(16))) __bf16 dzt[MLP_H][32]; /* [out unit][sample]               */
+    __shared__ __attribute__((aligned(16))) __bf16 h1t[MLP_H][32]; /* [in unit][h1 of 16 samples, g1 of 16]   */
+    __shared__ __attribute__((aligned(16))) __bf16 dzt[MLP_H][32]; /* [out unit][dz2 of 16 samples, dt2 of 16] */
     const int lane = threadIdx.x, n = lane & 15, g = lane >> 4;
     mlp_f4 dw2[16];
     float dw1a[16], dw1b[16], db2[16];
@@ -146,90 +160,97 @@ __global__ __launch_bounds__(64) void k_mlp_grad(struct mlp_net N, const float *
         dw2[q] = (mlp_f4){0.f, 0.f, 0.f, 0.f};
         dw1a[q] = dw1b[q] = db2[q] = 0.f;
     }
-    const int ntp = (ntiles + 1) / 2;
-    for (int tp = blockIdx.x; tp < ntp; tp += gridDim.x) {
-        for (int sub = 0; sub < 2; sub++) {
-            const int ti = 2 * tp + sub;
-            const bool valid = ti < ntiles;
-            const int tile = tiles[valid ? ti : 0];
-            const int s = 16 * tile + n;
-            int p = sp[16 * tile];
-            if ((unsigned)p >= (unsigned)np) {
-                if (lane == 0) *err = 1.0f;
-                p = 0;
-            }
-            float *trow = tpart + (size_t)ti * MLP_TROW;
-            const float r = sr[s];
-            float z1[16], z2[16], h2[16], y0, y1;
-            mlp_bf16x8 h1b[2];
-            mlp_forward16(N, r, p, lane, z1, h1b, z2, h2, y0, y1);
-            const float4 sc = N.sc[p];
-            const float d0 = y0 - asinhf(tE[s] / sc.z), d1 = y1 - asinhf(tF[s] / sc.w);
-            {
-                const float lt = mlp_sum16(valid && g == 0 ? wE * d0 * d0 + wF * d1 * d1 : 0.f);
-                if (valid && lane == 0) trow[MLP_TR_LOSS] = lt * inv_n;
-            }
-            const float dy0 = valid ? 2.0f * wE * d0 * inv_n : 0.0f;
-            const float dy1 = valid ? 2.0f * wF * d1 * inv_n : 0.0f;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const int tile = tiles[ti];
+        const int s = 16 * tile + n;
+        int p = sp[16 * tile];
+        if ((unsigned)p >= (unsigned)np) {
+            if (lane == 0) *err = 1.0f;
+            p = 0;
+        }
+        float *trow = tpart + (size_t)ti * MLP_TROW;
+        const float r = sr[s];
+        float z1[16], t1[16], z2[16], t2[16], y, yd;
+        mlp_bf16x8 h1b[2], g1b[2];
+        mlp_forward16(N, r, p, lane, z1, t1, h1b, g1b, z2, t2, y, yd);
+        const float4 sc = N.sc[p];
+        const float E = tE[s], F = tF[s];
+        const float c = 0.5f * (sc.y - sc.x);
+        const float ad = F / sqrtf(sc.z * sc.z + E * E);
+        const float d0 = y - asinhf(E / sc.z);
+        const float d1 = yd - ad;
+        const float q = 1.0f / (ad * ad + 1.0f / (c * c));
+        {
+            const float lt = mlp_sum16(g == 0 ? wE * d0 * d0 + wF * q * d1 * d1 : 0.f);
+            if (lane == 0) trow[MLP_TR_LOSS] = lt * inv_n;
+        }
+        const float dy = 2.0f * wE * d0 * inv_n;
+        const float dyd = 2.0f * wF * q * d1 * inv_n;
 
-            /* Heads: dW3[p], db3[p]; dZ2. */
-            const float *w3 = N.w3 + (size_t)p * 2 * MLP_H;
-            float dz2[16];
+        /* Head: dW3[p], db3[p]; dZ2, dT2. */
+        const float *w3 = N.w3 + (size_t)p * MLP_H;
+        float dz2[16], dt2[16];
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int u = mlp_unit(j, lane);
-                const float dh2 = dy0 * w3[u] + dy1 * w3[MLP_H + u];
-                dz2[j] = dh2 * mlp_dsilu(z2[j]);
-                db2[j] += dz2[j];
-                const float ga = mlp_sum16(dy0 * h2[j]);
-                const float gb = mlp_sum16(dy1 * h2[j]);
-                if (valid && n == 0) {
-                    trow[u] = ga;
-                    trow[MLP_H + u] = gb;
-                }
-            }
-            {
-                const float ga = mlp_sum16(dy0), gb = mlp_sum16(dy1);
-                if (valid && lane == 0) {
-                    trow[MLP_TR_B3] = ga;
-                    trow[MLP_TR_B3 + 1] = gb;
-                }
-            }
+        for (int j = 0; j < 16; j++) {
+            const int u = mlp_unit(j, lane);
+            const float w = w3[u];
+            float h, d, dd;
+            mlp_silu3(z2[j], h, d, dd);
+            dz2[j] = w * (dy * d + dyd * t2[j] * dd);
+            dt2[j] = w * dyd * d;
+            db2[j] += dz2[j];
+            const float gw = mlp_sum16(dy * h + dyd * d * t2[j]);
+            if (n == 0) trow[u] = gw;
+        }
+        {
+            const float gb = mlp_sum16(dy);
+            if (lane == 0) trow[MLP_TR_B3] = gb;
+        }
 
-            /* dH1 = W2^T dZ2 on MFMA; dZ1. */
-            mlp_bf16x8 dzb[2];
+        /* dH1 = W2^T dZ2 and dG1 = W2^T dT2 on MFMA; dZ1, dT1. */
+        mlp_bf16x8 dzb[2], dtb[2];
 #pragma unroll
-            for (int kb = 0; kb < 2; kb++)
+        for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-                for (int e = 0; e < 8; e++) dzb[kb][e] = (__bf16)dz2[8 * kb + e];
-            float rn, irn;
-            mlp_inputs(sc, r, rn, irn);
-#pragma unroll
-            for (int mb = 0; mb < 4; mb++) {
-                mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2b[(mb * 2 + 0) * 64 + lane], dzb[0], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(N.w2b[(mb * 2 + 1) * 64 + lane], dzb[1], acc, 0, 0, 0);
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int j = 4 * mb + i;
-                    const float dz1 = acc[i] * mlp_dsilu(z1[j]);
-                    dw1a[j] += dz1 * rn;
-                    dw1b[j] += dz1 * irn;
-                    const float gsum = mlp_sum16(dz1);
-                    if (valid && n == 0) trow[MLP_TR_B1E + mlp_unit(j, lane)] = gsum;
-                }
+            for (int e = 0; e < 8; e++) {
+                dzb[kb][e] = (__bf16)dz2[8 * kb + e];
+                dtb[kb][e] = (__bf16)dt2[8 * kb + e];
             }
-
-            /* Stage H1 and dZ2 (bf16) for dW2. */
+        float rn, irn, drn, dirn;
+        mlp_inputs(sc, r, rn, irn, drn, dirn);
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int u = mlp_unit(j, lane);
-                h1t[u][16 * sub + n] = h1b[j >> 3][j & 7];
-                dzt[u][16 * sub + n] = dzb[j >> 3][j & 7];
+        for (int mb = 0; mb < 4; mb++) {
+            const mlp_bf16x8 a0 = N.w2b[(mb * 2 + 0) * 64 + lane], a1 = N.w2b[(mb * 2 + 1) * 64 + lane];
+            mlp_f4 acc = {0.f, 0.f, 0.f, 0.f}, act = {0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, dzb[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, dzb[1], acc, 0, 0, 0);
+            act = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, dtb[0], act, 0, 0, 0);
+            act = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, dtb[1], act, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int j = 4 * mb + i;
+                float h, d, dd;
+                mlp_silu3(z1[j], h, d, dd);
+                const float dz1 = acc[i] * d + act[i] * t1[j] * dd;
+                const float dt1 = act[i] * d;
+                dw1a[j] += dz1 * rn + dt1 * drn;
+                dw1b[j] += dz1 * irn + dt1 * dirn;
+                const float gsum = mlp_sum16(dz1);
+                if (n == 0) trow[MLP_TR_B1E + mlp_unit(j, lane)] = gsum;
             }
         }
+
+        /* Stage H1|G1 and dZ2|dT2 (bf16) for dW2. */
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int u = mlp_unit(j, lane);
+            h1t[u][n] = h1b[j >> 3][j & 7];
+            h1t[u][16 + n] = g1b[j >> 3][j & 7];
+            dzt[u][n] = dzb[j >> 3][j & 7];
+            dzt[u][16 + n] = dtb[j >> 3][j & 7];
+        }
         __syncthreads();
-        /* dW2[m][k] += sum over the 32 samples of dZ2[m] H1[k]. */
+        /* dW2[m][k] += sum over the 32 columns of dZ2|dT2[m] H1|G1[k]. */
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) {
             const mlp_bf16x8 a = *(const mlp_bf16x8 *)&dzt[16 * mb + n][8 * g];
@@ -310,9 +331,9 @@ __global__ __launch_bounds__(256) void k_mlp_tsum(const float *__restrict__ tpar
             for (int t = 0; t < ntiles; t++)
                 if (mine[t]) acc += tpart[(size_t)t * MLP_TROW + c];
             if (c < MLP_TR_B3)
-                grad[MLP_OFF_W3(np) + p * 2 * MLP_H + c] += acc;
-            else if (c < MLP_TR_B1E)
-                grad[MLP_OFF_B3(np) + 2 * p + c - MLP_TR_B3] += acc;
+                grad[MLP_OFF_W3(np) + p * MLP_H + c] += acc;
+            else if (c == MLP_TR_B3)
+                grad[MLP_OFF_B3(np) + p] += acc;
             else
                 gb1e[p * MLP_H + c - MLP_TR_B1E] += acc;
         }
@@ -386,15 +407,25 @@ __global__ __launch_bounds__(64) void k_mlp_eval(struct mlp_net N, const float *
         const long long sv = s < n ? s : n - 1;
         const int p = pair[sv];
         const float rr = r[sv];
-        float y0, y1;
-        mlp_infer16(N, rr, p, lane, y0, y1);
+        float y, yd;
+        mlp_infer16(N, rr, p, lane, y, yd);
         if (lane < 16 && s < n) {
-            float e, f;
-            mlp_energy_force(N.sc[p], rr, y0, y1, e, f);
+            float e, dedr;
+            mlp_energy_force(N.sc[p], N.shift[p], rr, y, yd, e, dedr);
             E[s] = e;
-            F[s] = f * rr;
+            F[s] = dedr;
         }
     }
+}
+
+/* y of every pair at its r_hi (for the energy shifts). */
+__global__ __launch_bounds__(64) void k_mlp_rhi(struct mlp_net N, int np, float *__restrict__ yout) {
+    const int lane = threadIdx.x;
+    const int q = 16 * blockIdx.x + (lane & 15);
+    const int p = q < np ? q : np - 1;
+    float y, yd;
+    mlp_infer16(N, N.sc[p].y, p, lane, y, yd);
+    if (lane < 16 && q < np) yout[q] = y;
 }
 
 /* ------------------------------------------------------------ host side */
@@ -423,6 +454,11 @@ void gpu_mlp_free(struct gpu_state *g) {
     (void)hipFree(g->d_w2f);
     (void)hipFree(g->d_w2b);
     (void)hipFree(g->d_msc);
+    (void)hipFree(g->d_msh);
+    (void)hipFree(g->d_mshy);
+    free(g->h_msc);
+    g->h_msc = NULL;
+    g->d_msh = g->d_mshy = NULL;
     (void)hipFree(g->d_wpart);
     (void)hipFree(g->d_tpart);
     (void)hipFree(g->d_err);
@@ -472,6 +508,43 @@ static int mlp_publish(struct gpu_state *g) {
     return engine_err_ok;
 }
 
+/* Energy shifts: shift[p] = sE sinh(y_p(r_hi)) - E_p(r_hi), with E_p the
+   FP64 table of species pair p (potential_eval_r), so that the network's
+   unswitched energy at r_hi is the table's.  Run after every change of the
+   parameters outside a training step. */
+static int mlp_update_shift(struct engine *e, struct gpu_state *g) {
+    const int np = g->mlp_np;
+    float *y = (float *)malloc(sizeof(float) * np), *sh = (float *)malloc(sizeof(float) * np);
+    int p, rc = engine_err_ok;
+    if (y == NULL || sh == NULL) {
+        free(y);
+        free(sh);
+        return gerror_(engine_err_malloc, "malloc failed");
+    }
+    hipLaunchKernelGGL(k_mlp_rhi, dim3((np + 15) / 16), dim3(64), 0, g->stream, gpu_mlp_net(g), np, g->d_mshy);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess ||
+        hipMemcpy(y, g->d_mshy, sizeof(float) * np, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = gerror_(engine_err_gpu, "energy shift evaluation failed");
+    for (p = 0; p < np && rc == engine_err_ok; p++) {
+        const struct potential *pot = e->p[g->h_pair_ti[p] * g->max_type + g->h_pair_tj[p]];
+        double ee = 0.0, ff = 0.0;
+        if (pot == NULL) {
+            rc = gerror_(engine_err_range, "no table for a species pair");
+            break;
+        }
+        /* the network is evaluated at r_hi rounded to FP32, which may lie
+           just past the table's last node b; the table is read at b */
+        const double r = (double)g->h_msc[p].y < pot->b ? (double)g->h_msc[p].y : pot->b;
+        if (potential_eval_r(pot, &r, 1, &ee, &ff) < 0) rc = gerror_(engine_err_range, "table evaluation failed");
+        sh[p] = (float)(g->h_msc[p].z * sinh((double)y[p]) - ee);
+    }
+    if (rc == engine_err_ok && hipMemcpy(g->d_msh, sh, sizeof(float) * np, hipMemcpyHostToDevice) != hipSuccess)
+        rc = gerror_(engine_err_gpu, "energy shift upload failed");
+    free(y);
+    free(sh);
+    return rc;
+}
+
 /* Grow the device sample buffers to hold n samples (and n/16 tile ids). */
 static int mlp_samples(struct gpu_state *g, long long n) {
     if (n <= g->d_samples_cap) return engine_err_ok;
@@ -502,19 +575,19 @@ extern "C" int engine_mlp_pairs(struct engine *e, int *ti, int *tj, int max) {
 }
 
 extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const double *r_hi, const double *escale,
-                               const double *fscale, unsigned long long seed) {
+                               const double *rswitch, unsigned long long seed) {
     struct gpu_state *g = mlp_state(e, 0);
     int np, nparams, k;
     float *theta;
     float4 *sc;
     if (g == NULL) return engine_err;
-    if (r_lo == NULL || r_hi == NULL || escale == NULL || fscale == NULL)
+    if (r_lo == NULL || r_hi == NULL || escale == NULL || rswitch == NULL)
         return gerror_(engine_err_null, "NULL pair scales");
     np = g->ntab;
     if (np < 1) return gerror_(engine_err_range, "no potentials");
     for (k = 0; k < np; k++)
-        if (!(r_lo[k] > 0.0 && r_hi[k] > r_lo[k] && escale[k] > 0.0 && fscale[k] > 0.0))
-            return gerror_(engine_err_range, "pair scales must satisfy 0 < r_lo < r_hi, scales > 0");
+        if (!(r_lo[k] > 0.0 && r_hi[k] > r_lo[k] && escale[k] > 0.0 && rswitch[k] > r_lo[k] && isfinite(rswitch[k])))
+            return gerror_(engine_err_range, "pair scales must satisfy 0 < r_lo < r_hi, r_lo < r_switch, escale > 0");
     gpu_mlp_free(g);
     nparams = MLP_NPARAMS(np);
     theta = (float *)malloc(sizeof(float) * nparams);
@@ -532,9 +605,9 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
         for (k = 0; k < MLP_H * MLP_EMB; k++) theta[MLP_OFF_E1 + k] = (float)(mlp_normal(&s) * sqrt(1.0 / MLP_EMB));
         for (k = 0; k < MLP_H * MLP_H; k++) theta[MLP_OFF_W2 + k] = (float)(mlp_normal(&s) * sqrt(2.0 / MLP_H));
         for (k = 0; k < MLP_EMB * np; k++) theta[MLP_OFF_EMB + k] = (float)mlp_normal(&s);
-        for (k = 0; k < 2 * MLP_H * np; k++) theta[MLP_OFF_W3(np) + k] = (float)(mlp_normal(&s) * sqrt(1.0 / MLP_H));
+        for (k = 0; k < MLP_H * np; k++) theta[MLP_OFF_W3(np) + k] = (float)(mlp_normal(&s) * sqrt(1.0 / MLP_H));
     }
-    for (k = 0; k < np; k++) sc[k] = make_float4((float)r_lo[k], (float)r_hi[k], (float)escale[k], (float)fscale[k]);
+    for (k = 0; k < np; k++) sc[k] = make_float4((float)r_lo[k], (float)r_hi[k], (float)escale[k], (float)rswitch[k]);
     g->mlp_np = np;
     g->mlp_nparams = nparams;
     g->mlp_step = 0;
@@ -547,6 +620,8 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
         hipMalloc(&g->d_w2f, sizeof(mlp_bf16x8) * 512) != hipSuccess ||
         hipMalloc(&g->d_w2b, sizeof(mlp_bf16x8) * 512) != hipSuccess ||
         hipMalloc(&g->d_msc, sizeof(float4) * np) != hipSuccess ||
+        hipMalloc(&g->d_msh, sizeof(float) * np) != hipSuccess ||
+        hipMalloc(&g->d_mshy, sizeof(float) * np) != hipSuccess ||
         hipMalloc(&g->d_wpart, sizeof(float) * MLP_WPART * MLP_GRID_MAX) != hipSuccess ||
         hipMalloc(&g->d_tpart, sizeof(float) * MLP_TROW * MLP_TCHUNK) != hipSuccess ||
         hipMalloc(&g->d_err, sizeof(float)) != hipSuccess ||
@@ -558,13 +633,13 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
         return gerror_(engine_err_gpu, "MLP allocation failed");
     }
     free(theta);
-    free(sc);
+    g->h_msc = sc;
     hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_adam_m, (long long)nparams, 0.0f);
     hipLaunchKernelGGL(k_mlp_fill, dim3(64), dim3(256), 0, g->stream, g->d_adam_v, (long long)nparams, 0.0f);
     hipLaunchKernelGGL(k_mlp_fill, dim3(1), dim3(64), 0, g->stream, g->d_err, 1LL, 0.0f);
     if (mlp_publish(g) < 0) return engine_err_gpu;
     if (scratch_init(g->device) < 0) return engine_err_gpu;
-    HIPCHECK(hipStreamSynchronize(g->stream));
+    if (mlp_update_shift(e, g) < 0) return engine_err;
     return nparams;
 }
 
@@ -578,7 +653,7 @@ extern "C" int engine_mlp_params(struct engine *e, float *theta, int get) {
     } else {
         HIPCHECK(hipMemcpy(g->d_theta, theta, sizeof(float) * g->mlp_nparams, hipMemcpyHostToDevice));
         if (mlp_publish(g) < 0) return engine_err_gpu;
-        HIPCHECK(hipStreamSynchronize(g->stream));
+        if (mlp_update_shift(e, g) < 0) return engine_err;
     }
     return g->mlp_nparams;
 }
@@ -612,8 +687,7 @@ static int mlp_gradient(struct gpu_state *g, const struct mlp_bufs *B, const flo
     hipLaunchKernelGGL(k_mlp_fill, dim3(16), dim3(256), 0, g->stream, B->gb1e, (long long)np * MLP_H, 0.0f);
     for (int c0 = 0; c0 < ntiles; c0 += MLP_TCHUNK) {
         const int nt = ntiles - c0 < MLP_TCHUNK ? ntiles - c0 : MLP_TCHUNK;
-        const int ntp = (nt + 1) / 2;
-        const int grid = ntp < B->grid_max ? ntp : B->grid_max;
+        const int grid = nt < B->grid_max ? nt : B->grid_max;
         hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64), 0, g->stream, gpu_mlp_net(g), sr, sp, sE, sF, tiles + c0,
                            nt, wE, wF, inv_n, np, B->tpart, B->wpart, B->err);
         hipLaunchKernelGGL(k_mlp_wsum, dim3((MLP_WPART + 63) / 64), dim3(64 * MLP_WSUM_SPLIT), 0, g->stream,
@@ -827,7 +901,7 @@ extern "C" int engine_mlp_train(struct engine *e, const float *r, const int *pai
     HIPCHECK(hipStreamSynchronize(g->stream));
     HIPCHECK(hipGetLastError());
     if (loss_hist) HIPCHECK(hipMemcpy(loss_hist, g->d_loss, sizeof(float) * steps, hipMemcpyDeviceToHost));
-    return engine_err_ok;
+    return mlp_update_shift(e, g);
 }
 
 extern "C" int engine_mlp_eval(struct engine *e, const float *r, const int *pair, long long n, float *E, float *F) {

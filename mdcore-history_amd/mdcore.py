@@ -499,8 +499,10 @@ class Engine:
         check(self.lib.engine_mlp_pairs(self.e, ptr(ti, C.c_int), ptr(tj, C.c_int), n), "engine_mlp_pairs")
         return ti[:n], tj[:n]
 
-    def mlp_init(self, r_lo, r_hi, escale, fscale, seed=1):
-        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (r_lo, r_hi, escale, fscale)]
+    def mlp_init(self, r_lo, r_hi, escale, rswitch, seed=1):
+        """rswitch[p] >= r_hi[p]: no switch; otherwise the pair's energy is
+        switched off between rswitch[p] and r_hi[p]."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (r_lo, r_hi, escale, rswitch)]
         return check(self.lib.engine_mlp_init(self.e, *[ptr(v) for v in a], seed), "engine_mlp_init")
 
     def mlp_params(self, theta=None):

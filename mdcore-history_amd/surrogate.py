@@ -5,12 +5,15 @@ pairs inside each species pair's domain [r_lo, rc]; the tables stay the
 measure (targets, force error) and the fallback (pairs below r_lo).
 
   pair_domains   per species pair: r_lo where |E| first exceeds a cap
-                 (the repulsive core stays on the table), r_hi = rc, and the
-                 output scales of E = sE sinh(y0), dE/dr = sF sinh(y1)
+                 (the repulsive core stays on the table), r_hi = rc, the
+                 energy scale sE of E = sE sinh(y) and the switch start
+                 (r_hi: no switch)
   make_samples   half uniform in r per species pair, half the pair
                  distances of configurations (the plain engine's own
                  trajectory), targets E and dE/dr from the FP64 tables
   train          engine_mlp_init + engine_mlp_train on the GPU
+  nve_drift      energy drift of device-resident NVE steps with the table
+                 or with a network trained on the equilibrated state
 """
 import numpy as np
 
@@ -27,8 +30,10 @@ def pair_potentials(eng, sysm):
     return [table[(int(a), int(b))] for a, b in zip(ti, tj)], ti, tj
 
 
-def pair_domains(eng, sysm, ecap=1.0e4, escale=1.0, fscale=10.0, ngrid=20000):
-    """r_lo[p]: smallest r such that |E| <= ecap on [r, rc] (the table's a at least)."""
+def pair_domains(eng, sysm, ecap=1.0e4, escale=1.0, switch=None, ngrid=20000):
+    """r_lo[p]: smallest r such that |E| <= ecap on [r, rc] (the table's a at
+    least); switch: None, or the distance from which the network's energy is
+    switched off to zero at rc."""
     pots, _, _ = pair_potentials(eng, sysm)
     rc = float(sysm.cutoff)
     lo, hi = [], []
@@ -40,7 +45,8 @@ def pair_domains(eng, sysm, ecap=1.0e4, escale=1.0, fscale=10.0, ngrid=20000):
         lo.append(r[bad[-1] + 1] if len(bad) else a)
         hi.append(rc)
     n = len(pots)
-    return np.array(lo), np.array(hi), np.full(n, escale), np.full(n, fscale)
+    rsw = np.array(hi) if switch is None else np.full(n, float(switch))
+    return np.array(lo), np.array(hi), np.full(n, escale), rsw
 
 
 def configuration_pairs(eng, sysm, x, nmax, seed, r_lo):
@@ -90,8 +96,8 @@ def make_samples(eng, sysm, n, r_lo, r_hi, configs=(), seed=1):
 
 def train(eng, sysm, samples, domains, steps=4000, batch_tiles=512, lr0=3e-3, lr1=3e-5, seed=1):
     """engine_mlp_init + engine_mlp_train; returns the per-step loss."""
-    r_lo, r_hi, es, fs = domains
-    eng.mlp_init(r_lo, r_hi, es, fs, seed)
+    r_lo, r_hi, es, rsw = domains
+    eng.mlp_init(r_lo, r_hi, es, rsw, seed)
     r, p, E, F = samples
     return eng.mlp_train(r, p, E, F, steps, batch_tiles, lr0, lr1, 1.0, 1.0, seed)
 
@@ -120,3 +126,42 @@ def drift_per_ns(etot, dt):
     t = np.arange(len(etot)) * dt
     slope = np.polyfit(t, etot, 1)[0] * 1000.0
     return slope / abs(float(np.mean(etot))), slope
+
+
+def nve_drift(sysm, mlp=False, switch=None, equil=1000, steps=10000, dt=0.002, train_samples=1 << 20,
+              train_steps=10000, seed=7):
+    """Energy drift of `steps` device-resident NVE steps (engine_gpu_steps)
+    after `equil` steps with the system's tables.  With mlp, the run after
+    the equilibration evaluates the pairs with a network trained on the
+    tables (half uniform-in-r samples, half the equilibrated configuration's
+    pair distances), switched off to zero from `switch` to the cutoff if
+    given.  Total energy at step n: PE(x_n) + (KE(v_{n-1/2}) + KE(v_{n+1/2}))/2;
+    drift = least-squares slope per ns, relative to |<E>| (whole run and
+    each half) and in kJ/mol."""
+    import dataclasses
+    import time
+    import systems
+    t0 = time.perf_counter()
+    eng = systems.build_engine(sysm, md.engine_flag_gpu)
+    eng.setdt(dt)
+    eng.gpu_steps(equil)
+    if mlp:
+        s_eq = dataclasses.replace(sysm, x=eng.positions(), v=eng.velocities())
+        eng.close()
+        eng = systems.build_engine(s_eq, md.engine_flag_gpu)
+        eng.setdt(dt)
+        dom = pair_domains(eng, s_eq, switch=switch)
+        S = make_samples(eng, s_eq, train_samples, dom[0], dom[1], configs=[s_eq.x], seed=5)
+        eng.mlp_init(*dom, seed=seed)
+        eng.mlp_train(*S, steps=train_steps, batch_tiles=512, lr0=1e-2, lr1=1e-5, seed=11)
+        eng.setflags(md.engine_flag_mlp, 0)
+    ep, ek = eng.gpu_steps(steps)
+    eng.close()
+    et = ep[1:] + 0.5 * (ek[:-1] + ek[1:])
+    rel, absd = drift_per_ns(et, dt)
+    h = len(et) // 2
+    return {"drift_rel_per_ns": rel, "drift_kJ_mol_per_ns": absd,
+            "drift_rel_per_ns_halves": [drift_per_ns(et[:h], dt)[0], drift_per_ns(et[h:], dt)[0]],
+            "e_mean": float(et.mean()), "e_std": float(et.std()),
+            "temperature_K": float(2 * ek.mean() / (3 * sysm.N * systems.KB)),
+            "wall_s": round(time.perf_counter() - t0, 1)}
