@@ -44,7 +44,10 @@ sys.path.insert(0, str(ROOT / "mdcore-history_amd"))
 
 FLOP_PER_PAIR = 46          # algorithmic FLOP per evaluated pair interaction
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 peak
-MLP_MFMA_FLOP = 2 * 64 * 64  # 64x64 layer on MFMA per network evaluation
+# 64x64 layer on MFMA per network evaluation: the values and their
+# r-derivatives, 2 x 64 x 64 multiply-adds each (the bf16 hi + lo split of the
+# inputs runs each product twice; that precision work is not counted)
+MLP_MFMA_FLOP = 2 * 2 * 64 * 64
 BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
 MI355X_CUS = 256            # the pair kernel's 5-wave build runs from 64 cells per CU up
 DT_PS = 0.002               # 2 fs time step for ns/day
@@ -399,8 +402,9 @@ def mlp_section(sysm, eng, args, f_table, f64, table_value):
                      "kernel": "k_nonbond_mlp" if args.mlp_fullshell else "k_nonbond_mlp_half + k_fold",
                      "evaluations_per_pass": evals,
                      "kernel_ms": round(ms["nonbond_ms"], 4),
-                     "note": "bf16 MFMA FLOP of the 64x64 layer per network evaluation; VALU work (SiLU, "
-                             "layer 1, heads, pair search) not counted"},
+                     "note": "bf16 MFMA FLOP of the 64x64 layer per network evaluation, values and "
+                             "r-derivatives (16,384; the hi + lo split that doubles the products executed is not "
+                             "counted); VALU work (SiLU, layer 1, head, pair search) not counted"},
         "_rmse_fp64": rel_rmse(f_mlp, f64) if f64 is not None else None,
     }
     if sysm.name == "mixture":

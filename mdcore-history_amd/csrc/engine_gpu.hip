@@ -24,9 +24,9 @@
  *               cell is staged in LDS in window order as packed records
  *               (x, y, z, key | index << 16 | type << tshift).  Each lane walks
  *               its share of the sorted window (runner_dopair.c:455-478),
- *               tests the cutoff (re-done in FP64 inside a thin band around
- *               rc^2) and keeps its hits in a lane-private LDS list, then
- *               evaluates them two at a time: FP32 table evaluation
+ *               keeps every record with r^2 up to the top of a thin band
+ *               around rc^2 in a lane-private LDS list, then evaluates them
+ *               two at a time, the band's pairs decided in FP64 first: FP32 table evaluation
  *               (potential_eval.h:67-104), forces summed in registers and
  *               folded into two-sum (hi, lo) accumulators per neighbour.
  *               Excluded pairs are subtracted afterwards as in
@@ -39,10 +39,13 @@
  *   k_fold      per atom, the groups' partial forces and the half-shell
  *               slots, added in a fixed order;
  *   k_reduce_a/b per-item FP64 energies/virials/counts -> totals, fixed order.
- * In the full-shell kernels every pair is evaluated by both of its cells
- * (once per owner) with bitwise-opposite dx, so the two owners see exactly
- * opposite forces; energy, virial and counts are taken on the task's i side
- * only.
+ * Staged neighbour records hold their positions moved into the owner's
+ * frame (x + shift, once per record), so a candidate's distance is one
+ * subtraction per axis.  In the full-shell kernels every pair is evaluated
+ * by both of its cells (once per owner); the two owners' FP32 distances are
+ * opposite to rounding (each side rounds its own shifted position), so the
+ * forces sum to zero to FP32 rounding; energy, virial and counts are taken
+ * on the task's i side only.
  */
 #include <hip/hip_runtime.h>
 
@@ -714,6 +717,13 @@ __device__ __forceinline__ void nonbond_body(
     float *jf = (float *)(smem + P.lds_jf);                              /* [4][3*maxn] */
     const int lane = threadIdx.x;
     const int mt = P.max_type;
+    /* A candidate beyond rc2hi is out.  rc2hi is rc2f + band rounded down:
+       then every kept r2 has r2 - rc2f <= band exactly (the difference of
+       two floats this close is exact), so the evaluation's band test
+       catches every kept pair that is not certainly inside, and FP64
+       decides it.  (Rounded up, a pair at r2 = rc2hi could slip past both.) */
+    float rc2hi = P.rc2f + P.band;
+    if (rc2hi - P.rc2f > P.band) rc2hi = __uint_as_float(__float_as_uint(rc2hi) - 1u);
     const __amdgpu_buffer_rsrc_t rs_ptab = make_rsrc(ptab, P.ptab_bytes);
     const __amdgpu_buffer_rsrc_t rs_coef = make_rsrc(coeffs4, P.coef_bytes);
 
@@ -845,11 +855,20 @@ __device__ __forceinline__ void nonbond_body(
                 const int nj = nA + nB;
                 const int sid = en.y & 15, orient = (en.y >> 4) & 1, self = (en.y >> 5) & 1;
 
+                /* Shifts from the owner cell to the neighbours (minimum image),
+                   rounded to FP32 on the host; the FP64 value is recomputed
+                   where needed. */
+                const float4 sfv = nbs[k0 + k];
+                const float4 sgv = paired ? nbs[k0 + k + 1] : sfv;
+
                 /* Stage the neighbours, each in the order its window is
                    walked: ascending keys when the owner is the task's i cell,
                    descending otherwise, storage order for the self task.
-                   Record = (x, y, z, key' | index << 16 | type << tshift) with
-                   key' = key, or 0xffff - key when the owner is the j cell. */
+                   Record = (x + s, y + s, z + s, key' | index << 16 | type << tshift):
+                   the position moved into the owner's frame by its cell's
+                   shift s, so that a candidate's distance is one subtraction
+                   per axis, and key' = key, or 0xffff - key when the owner is
+                   the j cell. */
                 wave_lds_sync();
                 if (lane < nj) raw[lane] = pfx;
                 wave_lds_sync();
@@ -857,9 +876,10 @@ __device__ __forceinline__ void nonbond_body(
                     const bool sB = t >= nA;
                     const int u = sB ? t - nA : t, ns = sB ? nB : nA, os = sB ? offB : offj, rb = sB ? nA : 0;
                     const int ori = sB ? 1 : orient;
+                    const float4 sh = sB ? sgv : sfv;
                     if (self) {
                         const float4 p = t < 64 ? pfx : xf[offj + t];
-                        stage[t] = make_float4(p.x, p.y, p.z,
+                        stage[t] = make_float4(p.x + sh.x, p.y + sh.y, p.z + sh.z,
                                                __uint_as_float(((unsigned int)t << 16) |
                                                                ((unsigned int)__float_as_int(p.w) << P.tshift)));
                     } else {
@@ -868,7 +888,7 @@ __device__ __forceinline__ void nonbond_body(
                         const int idx = (int)(w >> 16) < ns ? (int)(w >> 16) : ns - 1;
                         const float4 p = rb + idx < 64 ? raw[rb + idx] : xf[os + idx];
                         const unsigned int kk = ori ? 0xffffu - (w & 0xffffu) : (w & 0xffffu);
-                        stage[t] = make_float4(p.x, p.y, p.z,
+                        stage[t] = make_float4(p.x + sh.x, p.y + sh.y, p.z + sh.z,
                                                __uint_as_float(kk | ((unsigned int)idx << 16) |
                                                                ((unsigned int)__float_as_int(p.w) << P.tshift)));
                     }
@@ -878,19 +898,6 @@ __device__ __forceinline__ void nonbond_body(
                 /* Prefetch the next visit while this one is processed. */
                 if (k + step < nn) prefetch(k + step);
                 if (nj == 0) continue;
-
-                /* Shifts from the owner cell to the neighbours (minimum image),
-                   rounded to FP32 on the host; the FP64 value is recomputed
-                   where needed. */
-                const float4 sfv = nbs[k0 + k];
-                const float sf0 = sfv.x, sf1 = sfv.y, sf2 = sfv.z;
-                float sg0 = 0.f, sg1 = 0.f, sg2 = 0.f;
-                if (paired) {
-                    const float4 sb = nbs[k0 + k + 1];
-                    sg0 = sb.x;
-                    sg1 = sb.y;
-                    sg2 = sb.z;
-                }
 
                 int myk = 0, dns = 0;
                 bool live_ref = true, run = true;
@@ -935,6 +942,24 @@ __device__ __forceinline__ void nonbond_body(
                 float lfx = 0.f, lfy = 0.f, lfz = 0.f, le = 0.f, lv = 0.f;
                 unsigned int lnp = 0, lnref = 0;
 
+                /* The reference's FP64 cutoff test (runner_dopair.c:465-489) of
+                   this lane's atom and the record with word w of cell cn
+                   (storage offset on), in the task's own orientation: the
+                   owner is the task's i cell (own_i) or its j cell. */
+                auto in_cut64 = [&](int cn, int on, unsigned int w, bool own_i) -> bool {
+                    const double *xa = &x64[3 * (size_t)a];
+                    const double *xb = &x64[3 * (size_t)(on + rec_idx(w, P.imask))];
+                    const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cn];
+                    const double r2d =
+                        own_i ? r2_task64(xa, xb, min_image(o_nb[0] - o_own[0], P.dim[0]),
+                                          min_image(o_nb[1] - o_own[1], P.dim[1]),
+                                          min_image(o_nb[2] - o_own[2], P.dim[2]))
+                              : r2_task64(xb, xa, min_image(o_own[0] - o_nb[0], P.dim[0]),
+                                          min_image(o_own[1] - o_nb[1], P.dim[1]),
+                                          min_image(o_own[2] - o_nb[2], P.dim[2]));
+                    return r2d <= P.cutoff2;
+                };
+
                 /* One visit, specialised by kind: 0 self task, 1 owner is the
                    task's i cell (energies, virial and counts are taken here),
                    2 owner is the j cell (forces only). */
@@ -954,28 +979,18 @@ __device__ __forceinline__ void nonbond_body(
                             /* half shell: a self task takes each pair once, j < i */
                             if (HALF && VK == 0) alive = alive && t < myidx;
                             const bool cand = VK == 0 ? (alive && t != myidx) : alive;
-                            const float dx = (xi.x - pj.x) - sf0, dy = (xi.y - pj.y) - sf1, dz = (xi.z - pj.z) - sf2;
+                            const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
                             const float r2 = dx * dx + dy * dy + dz * dz;
-                            bool hit = cand && r2 <= P.rc2f;
-                            const bool near = cand && fabsf(r2 - P.rc2f) <= P.band;
-                            if (wave_any(near)) {
-                                if (near) {
-                                    /* Decide in FP64 exactly as the reference does, in
-                                       the task's own orientation. */
-                                    const double *xa = &x64[3 * (size_t)a];
-                                    const double *xb = &x64[3 * (size_t)(offj + rec_idx(wj, P.imask))];
-                                    const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cj];
-                                    double r2d;
-                                    if (VK != 2)
-                                        r2d = r2_task64(xa, xb, min_image(o_nb[0] - o_own[0], P.dim[0]),
-                                                        min_image(o_nb[1] - o_own[1], P.dim[1]),
-                                                        min_image(o_nb[2] - o_own[2], P.dim[2]));
-                                    else
-                                        r2d = r2_task64(xb, xa, min_image(o_own[0] - o_nb[0], P.dim[0]),
-                                                        min_image(o_own[1] - o_nb[1], P.dim[1]),
-                                                        min_image(o_own[2] - o_nb[2], P.dim[2]));
-                                    hit = r2d <= P.cutoff2;
-                                }
+                            bool hit;
+                            if constexpr (MLP) {
+                                hit = cand && r2 <= P.rc2f;
+                                const bool near = cand && fabsf(r2 - P.rc2f) <= P.band;
+                                if (wave_any(near))
+                                    if (near) hit = in_cut64(cj, offj, wj, VK != 2);
+                            } else {
+                                /* keep everything up to the band's upper edge; the
+                                   evaluation decides the band's pairs in FP64 */
+                                hit = cand && r2 <= rc2hi;
                             }
                             hitl[h * 64 + lane] = (unsigned short)t;
                             h += hit ? 1 : 0;
@@ -1000,14 +1015,19 @@ __device__ __forceinline__ void nonbond_body(
                                 const unsigned int wa = __float_as_uint(pa.w), wb = __float_as_uint(pb.w);
                                 const float4 ha = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wa, P.tshift)));
                                 const float4 hb = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wb, P.tshift)));
-                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
-                                            dza = (xi.z - pa.z) - sf2;
-                                const float dxb = (xi.x - pb.x) - sf0, dyb = (xi.y - pb.y) - sf1,
-                                            dzb = (xi.z - pb.z) - sf2;
+                                const float dxa = xi.x - pa.x, dya = xi.y - pa.y, dza = xi.z - pa.z;
+                                const float dxb = xi.x - pb.x, dyb = xi.y - pb.y, dzb = xi.z - pb.z;
                                 const float r2a = dxa * dxa + dya * dya + dza * dza;
                                 const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                                const bool ua = va && __float_as_int(ha.w) != -1;
-                                const bool ub = vb && __float_as_int(hb.w) != -1;
+                                bool ua = va && __float_as_int(ha.w) != -1;
+                                bool ub = vb && __float_as_int(hb.w) != -1;
+                                /* pairs within the band of the cutoff: FP64 decides */
+                                const bool na = ua && fabsf(r2a - P.rc2f) <= P.band;
+                                const bool nb = ub && fabsf(r2b - P.rc2f) <= P.band;
+                                if (wave_any(na || nb)) {
+                                    if (na) ua = in_cut64(cj, offj, wa, VK != 2);
+                                    if (nb) ub = in_cut64(cj, offj, wb, VK != 2);
+                                }
                                 float ea, fa, eb, fb;
                                 pot_eval32r2(ha, hb, ua, ub, rs_coef, r2a, r2b, ea, fa, eb, fb);
                                 lfx -= fa * dxa + fb * dxb;
@@ -1057,8 +1077,7 @@ __device__ __forceinline__ void nonbond_body(
                                 const unsigned int wa = __float_as_uint(pa.w);
                                 const float4 ha = ptab[ti * mt + rec_type(wa, P.tshift)];
                                 if (__float_as_int(ha.w) == -1) continue;
-                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
-                                            dza = (xi.z - pa.z) - sf2;
+                                const float dxa = xi.x - pa.x, dya = xi.y - pa.y, dza = xi.z - pa.z;
                                 const float r2a = dxa * dxa + dya * dya + dza * dza;
                                 const float rlo = N.sc[ptype[ti * mt + rec_type(wa, P.tshift)]].x;
                                 if (r2a >= rlo * rlo) {
@@ -1090,8 +1109,7 @@ __device__ __forceinline__ void nonbond_body(
                             const int T = __shfl(off + nd, 63);
                             for (int s = 0; s < nd; s++) {
                                 const float4 pa = stage[hitl[s * 64 + lane]];
-                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
-                                            dza = (xi.z - pa.z) - sf2;
+                                const float dxa = xi.x - pa.x, dya = xi.y - pa.y, dza = xi.z - pa.z;
                                 dn_r2[off + s] = dxa * dxa + dya * dya + dza * dza;
                                 dn_p[off + s] = ptype[ti * mt + rec_type(__float_as_uint(pa.w), P.tshift)];
                             }
@@ -1102,8 +1120,7 @@ __device__ __forceinline__ void nonbond_body(
                                 const int ta = hitl[s * 64 + lane];
                                 const float4 pa = stage[ta];
                                 const unsigned int wa = __float_as_uint(pa.w);
-                                const float dxa = (xi.x - pa.x) - sf0, dya = (xi.y - pa.y) - sf1,
-                                            dza = (xi.z - pa.z) - sf2;
+                                const float dxa = xi.x - pa.x, dya = xi.y - pa.y, dza = xi.z - pa.z;
                                 const float r2a = dxa * dxa + dya * dya + dza * dza;
                                 const float fa = dn_f[off + s], ea = dn_e[off + s];
                                 lfx -= fa * dxa;
@@ -1144,31 +1161,9 @@ __device__ __forceinline__ void nonbond_body(
                             const bool inwin = alive && t < ns && (int)rec_key(wj) < km;
                             const bool sw = alive && !inwin && seg == 0;
                             alive = inwin || (sw && runB);
-                            const float s0 = seg ? sg0 : sf0, s1 = seg ? sg1 : sf1, s2 = seg ? sg2 : sf2;
-                            const float dx = (xi.x - pj.x) - s0, dy = (xi.y - pj.y) - s1, dz = (xi.z - pj.z) - s2;
+                            const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
                             const float r2 = dx * dx + dy * dy + dz * dz;
-                            bool hit = inwin && r2 <= P.rc2f;
-                            const bool near = inwin && fabsf(r2 - P.rc2f) <= P.band;
-                            if (wave_any(near)) {
-                                if (near) {
-                                    /* Decide in FP64 exactly as the reference does, in
-                                       the task's own orientation. */
-                                    const int cn = seg ? enb.x : cj, on = seg ? offB : offj;
-                                    const double *xa = &x64[3 * (size_t)a];
-                                    const double *xb = &x64[3 * (size_t)(on + rec_idx(wj, P.imask))];
-                                    const double *o_own = &origin[3 * c], *o_nb = &origin[3 * cn];
-                                    double r2d;
-                                    if (seg == 0)
-                                        r2d = r2_task64(xa, xb, min_image(o_nb[0] - o_own[0], P.dim[0]),
-                                                        min_image(o_nb[1] - o_own[1], P.dim[1]),
-                                                        min_image(o_nb[2] - o_own[2], P.dim[2]));
-                                    else
-                                        r2d = r2_task64(xb, xa, min_image(o_own[0] - o_nb[0], P.dim[0]),
-                                                        min_image(o_own[1] - o_nb[1], P.dim[1]),
-                                                        min_image(o_own[2] - o_nb[2], P.dim[2]));
-                                    hit = r2d <= P.cutoff2;
-                                }
-                            }
+                            const bool hit = inwin && r2 <= rc2hi;
                             hitl[h * 64 + lane] = (unsigned short)tt;
                             h += hit ? 1 : 0;
                             if (sw) {
@@ -1188,14 +1183,18 @@ __device__ __forceinline__ void nonbond_body(
                             const float4 ha = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wa, P.tshift)));
                             const float4 hb = ld_f4(rs_ptab, 16 * (ti * mt + rec_type(wb, P.tshift)));
                             const bool ia = ta < nA, ib = tb < nA;
-                            const float dxa = (xi.x - pa.x) - (ia ? sf0 : sg0), dya = (xi.y - pa.y) - (ia ? sf1 : sg1),
-                                        dza = (xi.z - pa.z) - (ia ? sf2 : sg2);
-                            const float dxb = (xi.x - pb.x) - (ib ? sf0 : sg0), dyb = (xi.y - pb.y) - (ib ? sf1 : sg1),
-                                        dzb = (xi.z - pb.z) - (ib ? sf2 : sg2);
+                            const float dxa = xi.x - pa.x, dya = xi.y - pa.y, dza = xi.z - pa.z;
+                            const float dxb = xi.x - pb.x, dyb = xi.y - pb.y, dzb = xi.z - pb.z;
                             const float r2a = dxa * dxa + dya * dya + dza * dza;
                             const float r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-                            const bool ua = va && __float_as_int(ha.w) != -1;
-                            const bool ub = vb && __float_as_int(hb.w) != -1;
+                            bool ua = va && __float_as_int(ha.w) != -1;
+                            bool ub = vb && __float_as_int(hb.w) != -1;
+                            const bool na = ua && fabsf(r2a - P.rc2f) <= P.band;
+                            const bool nb = ub && fabsf(r2b - P.rc2f) <= P.band;
+                            if (wave_any(na || nb)) {
+                                if (na) ua = ia ? in_cut64(cj, offj, wa, true) : in_cut64(enb.x, offB, wa, false);
+                                if (nb) ub = ib ? in_cut64(cj, offj, wb, true) : in_cut64(enb.x, offB, wb, false);
+                            }
                             float ea, fa, eb, fb;
                             pot_eval32r2(ha, hb, ua, ub, rs_coef, r2a, r2b, ea, fa, eb, fb);
                             lfx -= fa * dxa + fb * dxb;

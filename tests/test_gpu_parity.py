@@ -144,7 +144,8 @@ def test_forces_energy_vs_fp64(case):
     assert abs(en["epot_nonbond"] - o["epot"]) <= ENERGY_RTOL * abs(o["epot"])
     assert abs(en["epot_exclusion"] - o["epot_exclusion"]) <= ENERGY_RTOL * max(1.0, abs(o["epot_exclusion"]))
     assert abs(en["virial"] - o["virial"]) <= ENERGY_RTOL * abs(o["virial"])
-    # Newton's third law: both cells of a pair see bitwise-opposite pair forces.
+    # Newton's third law: both cells of a pair see opposite pair forces (to
+    # FP32 rounding in the full shell, exactly in the half shell).
     assert np.abs(f.sum(0)).max() <= 1e-6 * np.abs(f).sum(0).max()
 
 
