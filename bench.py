@@ -51,6 +51,7 @@ MLP_MFMA_FLOP = 2 * 2 * 64 * 64
 BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
 MI355X_CUS = 256            # the pair kernel's 5-wave build runs from 64 cells per CU up
 DT_PS = 0.002               # 2 fs time step for ns/day
+MLP_TRAIN_STEPS = 40000     # surrogate training steps on the mixture (16,384 samples each)
 METRIC = "pair-interactions/s & ns/day at 1M atoms; MLP vs table at fixed force RMSE"
 TRAFFIC_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_traffic.json"
 COUNTERS_FILE = ROOT / "profiles" / "k_nonbond_mixture_1m_counters.json"
@@ -203,6 +204,7 @@ def main():
                          "equilibration, then --nve-steps steps with the tables or with a surrogate trained on 2^24 "
                          "samples of the equilibrated state (its own command: it takes minutes)")
     ap.add_argument("--nve-steps", type=int, default=10000)
+    ap.add_argument("--nve-dt", type=float, default=0.002, help="time step (ps) of --mixture-nve")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times, as the "
                          "known-size reference for the FETCH_SIZE / WRITE_SIZE counter scale")
@@ -219,7 +221,8 @@ def main():
         print(json.dumps({"md_step": md_step_section(args.md_steps)}), flush=True)
         return
     if args.mixture_nve:
-        print(json.dumps({"mixture_nve": mixture_nve(args.mixture_nve == "mlp", args.nve_steps)}), flush=True)
+        print(json.dumps({"mixture_nve": mixture_nve(args.mixture_nve == "mlp", args.nve_steps, dt=args.nve_dt)}),
+              flush=True)
         return
 
     import mdcore as md
@@ -364,7 +367,7 @@ def mlp_section(sysm, eng, args, f_table, f64, table_value):
     S = sg.make_samples(eng, sysm, 1 << 22, dom[0], dom[1], configs=[sysm.x], seed=5)
     t1 = time.perf_counter()
     eng.mlp_init(*dom, seed=7)
-    hist = eng.mlp_train(*S, steps=20000, batch_tiles=1024, lr0=1e-2, lr1=1e-5, seed=11)
+    hist = eng.mlp_train(*S, steps=MLP_TRAIN_STEPS, batch_tiles=1024, lr0=1e-2, lr1=1e-5, seed=11)
     t2 = time.perf_counter()
     eng.setflags(md.engine_flag_mlp | (md.engine_flag_fullshell if args.mlp_fullshell else 0), 0)
     eng.nonbond()
@@ -394,7 +397,7 @@ def mlp_section(sysm, eng, args, f_table, f64, table_value):
         "vs_table": res["pairs"] / (ms["total_ms"] * 1e-3) / table_value,
         "force_rmse_vs_table": rel_rmse(f_mlp, f_table),
         "pairs": res["pairs"], "pairs_below_r_lo_share": round(frac_table, 5),
-        "training": {"samples": int(len(S[0])), "steps": 20000, "batch": 1024 * 16,
+        "training": {"samples": int(len(S[0])), "steps": MLP_TRAIN_STEPS, "batch": 1024 * 16,
                      "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
                      "sample_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -535,7 +538,7 @@ def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 
         dom = sg.pair_domains(eng, s_eq)
         S = sg.make_samples(eng, s_eq, samples, dom[0], dom[1], configs=[s_eq.x], seed=5)
         eng.mlp_init(*dom, seed=7)
-        hist = eng.mlp_train(*S, steps=20000, batch_tiles=1024, lr0=1e-2, lr1=1e-5, seed=11)
+        hist = eng.mlp_train(*S, steps=MLP_TRAIN_STEPS, batch_tiles=1024, lr0=1e-2, lr1=1e-5, seed=11)
         note(f"surrogate trained: loss {hist[:20].mean():.3e} -> {hist[-50:].mean():.3e}")
         eng.setflags(md.engine_flag_mlp, 0)
         eng.nonbond()
@@ -547,7 +550,7 @@ def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 
         orc = po.from_system(s_eq)
         o = orc.nonbond(po.OR_WINDOW_REF, host_cpus()["threads"])
         orc.close()
-        out["training"] = {"samples": int(len(S[0])), "steps": 20000, "batch": 16384,
+        out["training"] = {"samples": int(len(S[0])), "steps": MLP_TRAIN_STEPS, "batch": 16384,
                            "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
                            "s": round(time.perf_counter() - t1, 1)}
         out["force_rel_rmse"] = {"mlp_vs_table": rel_rmse(f_mlp, f_table), "mlp_vs_fp64": rel_rmse(f_mlp, o["f"]),
@@ -709,13 +712,13 @@ def eng_cells(sysm):
 
 def pair_kernel_name(sysm, md):
     """The pair kernel engine_gpu_load picks: the half shell (with k_fold) for
-    engine_gpu_half_mean or more atoms per cell on average, the 5-wave build
-    from 64 cells per CU."""
+    engine_gpu_half_mean or more atoms per cell on average, otherwise the full
+    shell with paired visits; the 5-wave build from 64 cells per CU."""
     cells = eng_cells(sysm)
     wide = cells >= 64 * MI355X_CUS
     if sysm.N >= md.engine_gpu_half_mean * cells:
         return ("k_nonbond_half_wide" if wide else "k_nonbond_half") + " + k_fold"
-    return "k_nonbond_wide" if wide else "k_nonbond"
+    return "k_nonbond_paired_wide" if wide else "k_nonbond_paired"
 
 
 if __name__ == "__main__":
