@@ -205,6 +205,9 @@ def main():
                          "samples of the equilibrated state (its own command: it takes minutes)")
     ap.add_argument("--nve-steps", type=int, default=10000)
     ap.add_argument("--nve-dt", type=float, default=0.002, help="time step (ps) of --mixture-nve")
+    ap.add_argument("--nve-window", choices=["ref", "proj"], default="ref",
+                    help="pair window of --mixture-nve: the reference's sorted window, or the projected window "
+                         "that admits every pair inside the cutoff")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times, as the "
                          "known-size reference for the FETCH_SIZE / WRITE_SIZE counter scale")
@@ -221,7 +224,8 @@ def main():
         print(json.dumps({"md_step": md_step_section(args.md_steps)}), flush=True)
         return
     if args.mixture_nve:
-        print(json.dumps({"mixture_nve": mixture_nve(args.mixture_nve == "mlp", args.nve_steps, dt=args.nve_dt)}),
+        print(json.dumps({"mixture_nve": mixture_nve(args.mixture_nve == "mlp", args.nve_steps, dt=args.nve_dt,
+                                                      window=args.nve_window)}),
               flush=True)
         return
 
@@ -498,12 +502,15 @@ def nve_section(steps=10000, dt=0.002, equil=1000):
     return out
 
 
-def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 << 24):
+def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 << 24, window="ref"):
     """NVE of the 1,048,576-atom mixture with force-continuous tables (LJ
     switched off from `switch` to the 1.2 nm cutoff, conducting-boundary
     reaction field): device-resident equilibration with velocity rescaling
     (the reference drivers' scheme, examples/argon/argon.c:286-299) up to
-    dt, then `steps` NVE steps.  With mlp the NVE steps evaluate the pairs
+    dt, then `steps` NVE steps.  window "ref" evaluates the pairs of the
+    reference's sorted window (runner_dopair.c:455-478, which leaves out a
+    few pairs inside the cutoff at this cell size), "proj" every pair inside
+    the cutoff.  With mlp the NVE steps evaluate the pairs
     with a surrogate trained on `samples` samples (half uniform in r, half
     the equilibrated configuration's pair distances); its force and energy
     error against the table pass and the FP64 oracle are taken on the
@@ -516,7 +523,8 @@ def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 
     import systems
     t0 = time.perf_counter()
     sysm = systems.mixture(1048576, switch=switch)
-    eng = systems.build_engine(sysm, md.engine_flag_gpu)
+    flags = md.engine_flag_gpu | (md.engine_flag_projwindow if window == "proj" else 0)
+    eng = systems.build_engine(sysm, flags)
     stages = systems.MIXTURE_EQUILIBRATION + ((dt, 400, 100),)
     eq = systems.equilibrate(eng, sysm, T=T, stages=stages,
                              log=lambda d, tc, ep: None)
@@ -524,11 +532,12 @@ def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 
     out = {"system": "mixture_1m: 1,048,576 atoms, 20 species, LJ switched from "
                      f"{switch} nm + reaction field, rc 1.2 nm, 0.6 nm cells",
            "equilibration": [{"dt_ps": d, "T_K": tc, "epot": ep} for d, tc, ep in eq], "dt_ps": dt, "steps": steps,
+           "window": window,
            "pass": "surrogate" if mlp else "table"}
     if mlp:
         s_eq = dataclasses.replace(sysm, x=eng.positions(), v=eng.velocities())
         eng.close()
-        eng = systems.build_engine(s_eq, md.engine_flag_gpu)
+        eng = systems.build_engine(s_eq, flags)
         eng.setdt(dt)
         eng.shuffle()
         eng.nonbond()
@@ -548,7 +557,7 @@ def mixture_nve(mlp, steps, dt=0.002, switch=1.0, T=300.0, chunk=500, samples=1 
             sys.path.insert(0, str(ROOT / "oracle"))
         import pyoracle as po
         orc = po.from_system(s_eq)
-        o = orc.nonbond(po.OR_WINDOW_REF, host_cpus()["threads"])
+        o = orc.nonbond(po.OR_WINDOW_PROJ if window == "proj" else po.OR_WINDOW_REF, host_cpus()["threads"])
         orc.close()
         out["training"] = {"samples": int(len(S[0])), "steps": MLP_TRAIN_STEPS, "batch": 16384,
                            "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),

@@ -393,9 +393,68 @@ __device__ __forceinline__ unsigned int row0_max_u32(unsigned int v) {
     return (unsigned int)__builtin_amdgcn_readlane((int)v, 15);
 }
 
+/* One partition step of sort_desc_t on a[lo..hi] (hi - lo >= 15, at most
+   64 R elements) by a whole wave, pivot p = the middle key; returns the
+   scan's final (i, j).  ltab / rtab: hi - lo + 2 entries each. */
+template <int R>
+__device__ __forceinline__ int2 part_wave(unsigned int *a, int lo, int hi, unsigned short *ltab,
+                                          unsigned short *rtab, int lane) {
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned int p = a[(lo + hi) / 2] & 0xffffu;
+    unsigned int wv[R];
+    bool isl[R], isr[R];
+    int nR = 0;
+    for (int r = 0; r < R; r++) {
+        const int x = lo + lane + 64 * r;
+        const bool valid = x <= hi;
+        wv[r] = valid ? a[x] : 0u;
+        isl[r] = valid && (wv[r] & 0xffffu) <= p;
+        isr[r] = valid && (wv[r] & 0xffffu) >= p;
+        nR += __builtin_popcountll(__builtin_amdgcn_ballot_w64(isr[r]));
+    }
+    int cl = 0, cr = 0;
+    for (int r = 0; r < R; r++) {
+        const int x = lo + lane + 64 * r;
+        const unsigned long long ml = __builtin_amdgcn_ballot_w64(isl[r]);
+        const unsigned long long mr = __builtin_amdgcn_ballot_w64(isr[r]);
+        if (isl[r]) ltab[cl + __builtin_popcountll(ml & below) + 1] = (unsigned short)x;
+        if (isr[r]) rtab[nR - cr - __builtin_popcountll(mr & below)] = (unsigned short)x;
+        cl += __builtin_popcountll(ml);
+        cr += __builtin_popcountll(mr);
+    }
+    const int nL = cl, m = nL < nR ? nL : nR;
+    wave_lds_sync();
+    int K = 0;
+    int li[R], ri[R];
+    for (int r = 0; r < R; r++) {
+        const int k = 1 + lane + 64 * r;
+        li[r] = k <= m ? ltab[k] : 0;
+        ri[r] = k <= m ? rtab[k] : 0;
+        K += __builtin_popcountll(__builtin_amdgcn_ballot_w64(k <= m && li[r] <= ri[r]));
+    }
+    /* the K swaps touch distinct elements, each lane its own */
+    for (int r = 0; r < R; r++) {
+        const int k = 1 + lane + 64 * r;
+        if (k <= K && li[r] < ri[r]) {
+            const unsigned int va = a[li[r]], vb = a[ri[r]];
+            a[li[r]] = vb;
+            a[ri[r]] = va;
+        }
+    }
+    wave_lds_sync();
+    const int lK = ltab[K], rK = rtab[K];
+    int i = lK + 1, j = rK - 1;
+    if (i <= j) {
+        const int l1 = K + 1 <= nL ? (int)ltab[K + 1] : 1 << 30;
+        const int r1 = K + 1 <= nR ? (int)rtab[K + 1] : -(1 << 30);
+        i = l1 < rK ? l1 : rK;
+        j = r1 > lK ? r1 : lK;
+    }
+    return make_int2(i, j);
+}
+
 __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, unsigned short *ltab,
                                unsigned short *rtab, int *overflow, int lane) {
-    const unsigned long long below = (1ull << lane) - 1ull;
     int q = 0;
     if (lane == 0) {
         st_lo[0] = 0;
@@ -431,56 +490,8 @@ __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, u
             wave_lds_sync();
             continue;
         }
-        const unsigned int p = a[(lo + hi) / 2] & 0xffffu;
-        unsigned int wv[3];
-        bool isl[3], isr[3];
-        int nR = 0;
-        for (int r = 0; r < 3; r++) {
-            const int x = lo + lane + 64 * r;
-            const bool valid = x <= hi;
-            wv[r] = valid ? a[x] : 0u;
-            isl[r] = valid && (wv[r] & 0xffffu) <= p;
-            isr[r] = valid && (wv[r] & 0xffffu) >= p;
-            nR += __builtin_popcountll(__builtin_amdgcn_ballot_w64(isr[r]));
-        }
-        int cl = 0, cr = 0;
-        for (int r = 0; r < 3; r++) {
-            const int x = lo + lane + 64 * r;
-            const unsigned long long ml = __builtin_amdgcn_ballot_w64(isl[r]);
-            const unsigned long long mr = __builtin_amdgcn_ballot_w64(isr[r]);
-            if (isl[r]) ltab[cl + __builtin_popcountll(ml & below) + 1] = (unsigned short)x;
-            if (isr[r]) rtab[nR - cr - __builtin_popcountll(mr & below)] = (unsigned short)x;
-            cl += __builtin_popcountll(ml);
-            cr += __builtin_popcountll(mr);
-        }
-        const int nL = cl, m = nL < nR ? nL : nR;
-        wave_lds_sync();
-        int K = 0;
-        int li[3], ri[3];
-        for (int r = 0; r < 3; r++) {
-            const int k = 1 + lane + 64 * r;
-            li[r] = k <= m ? ltab[k] : 0;
-            ri[r] = k <= m ? rtab[k] : 0;
-            K += __builtin_popcountll(__builtin_amdgcn_ballot_w64(k <= m && li[r] <= ri[r]));
-        }
-        /* the K swaps touch distinct elements, each lane its own */
-        for (int r = 0; r < 3; r++) {
-            const int k = 1 + lane + 64 * r;
-            if (k <= K && li[r] < ri[r]) {
-                const unsigned int va = a[li[r]], vb = a[ri[r]];
-                a[li[r]] = vb;
-                a[ri[r]] = va;
-            }
-        }
-        wave_lds_sync();
-        const int lK = ltab[K], rK = rtab[K];
-        int i = lK + 1, j = rK - 1;
-        if (i <= j) {
-            const int l1 = K + 1 <= nL ? (int)ltab[K + 1] : 1 << 30;
-            const int r1 = K + 1 <= nR ? (int)rtab[K + 1] : -(1 << 30);
-            i = l1 < rK ? l1 : rK;
-            j = r1 > lK ? r1 : lK;
-        }
+        const int2 ij = part_wave<3>(a, lo, hi, ltab, rtab, lane);
+        const int i = ij.x, j = ij.y;
         if (q + 2 >= MD_SORT_STACK) {
             if (lane == 0) *overflow = 1;
             return;
@@ -509,18 +520,124 @@ __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, u
     }
 }
 
+/* Largest value over the lane's row of 16 lanes, in every lane of the row:
+   max with the row rotated by 1, 2, 4 and 8 lanes (DPP row_ror). */
+__device__ __forceinline__ unsigned int row_max_all(unsigned int v) {
+    unsigned int u;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xf, 0xf, false); /* row_ror:1 */
+    v = u > v ? u : v;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xf, 0xf, false); /* row_ror:2 */
+    v = u > v ? u : v;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xf, 0xf, false); /* row_ror:4 */
+    v = u > v ? u : v;
+    u = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xf, 0xf, false); /* row_ror:8 */
+    v = u > v ? u : v;
+    return v;
+}
+
+/* sort_desc_t on every list l of a cell with bit l of tm set (the lists of
+   n <= 64 words at a + l * n), by one wave, with the same result.  A
+   partition or a selection sort only touches its own segment, so the
+   segments are taken in any order:
+   - segments of 16 or more elements are partitioned one at a time
+     (part_wave), their parts queued again;
+   - segments of 2..15 elements (the serial sort's selection-sort leaves)
+     are sorted four at a time, one per row of 16 lanes: each step takes the
+     first maximal key of the rest, the row max of
+     key << 12 | (15 - row lane) << 8 | particle index
+     (particle indices stay below 256), and swaps it with the step's lane,
+     whose word every lane of the row gets from a second row max.
+   Segment queues: segq (segcap words) and leafq (leafcap words), lo | hi << 16
+   in list-array positions.  Reports a queue overflow or a runaway loop in
+   *overflow (the host turns it into a sort failure). */
+__device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigned int *segq, int segcap,
+                                unsigned int *leafq, int leafcap, unsigned short *ltab, unsigned short *rtab,
+                                int *overflow, int lane) {
+    int ns = 0, nl = 0;
+    bool bad = false;
+    auto push = [&](int lo, int hi) {
+        const unsigned int w = (unsigned int)lo | ((unsigned int)hi << 16);
+        if (hi - lo >= 15) {
+            if (ns < segcap) {
+                if (lane == 0) segq[ns] = w;
+                ns++;
+            } else
+                bad = true;
+        } else if (hi > lo) {
+            if (nl < leafcap) {
+                if (lane == 0) leafq[nl] = w;
+                nl++;
+            } else
+                bad = true;
+        }
+    };
+    for (unsigned int t = tm; t; t &= t - 1) {
+        const int l = __builtin_ctz(t);
+        push(l * n, l * n + n - 1);
+    }
+    wave_lds_sync();
+    /* each partition splits a segment into two strictly smaller ones */
+    int parts = 0;
+    while (ns > 0 && !bad) {
+        ns--;
+        const unsigned int sg = segq[ns];
+        const int lo = (int)(sg & 0xffffu), hi = (int)(sg >> 16);
+        if (++parts > 13 * n) {
+            bad = true;
+            break;
+        }
+        const int2 ij = part_wave<1>(a, lo, hi, ltab, rtab, lane);
+        push(lo, ij.y);
+        push(ij.x, hi);
+        wave_lds_sync();
+    }
+    if (bad) {
+        if (lane == 0) *overflow = 1;
+        return;
+    }
+    const int row = lane >> 4, lr = lane & 15;
+    for (int b = 0; b < nl; b += 4) {
+        const bool has = b + row < nl;
+        const unsigned int sg = has ? leafq[b + row] : 0u;
+        const int lo = (int)(sg & 0xffffu), len = has ? (int)(sg >> 16) - lo + 1 : 0;
+        unsigned int w = lr < len ? a[lo + lr] : 0u;
+        int steps = __builtin_amdgcn_readlane(len, 0);
+        steps = max(steps, __builtin_amdgcn_readlane(len, 16));
+        steps = max(steps, __builtin_amdgcn_readlane(len, 32));
+        steps = max(steps, __builtin_amdgcn_readlane(len, 48));
+        for (int i = 0; i < steps - 1; i++) {
+            const bool in = lr >= i && lr < len;
+            const unsigned int pk = in ? ((w & 0xffffu) << 12) | ((unsigned int)(15 - lr) << 8) | (w >> 16) : 0u;
+            const unsigned int mx = row_max_all(pk);
+            const unsigned int wi = row_max_all(lr == i ? w : 0u);
+            const int im = 15 - (int)((mx >> 8) & 15u);
+            const unsigned int wm = ((mx & 0xffu) << 16) | (mx >> 12);
+            if (i < len - 1) {
+                if (lr == i)
+                    w = wm;
+                else if (lr == im)
+                    w = wi;
+            }
+        }
+        if (lr < len) a[lo + lr] = w;
+        wave_lds_sync();
+    }
+}
+
 /* Sort by rank: one workgroup per cell, all 13 directions at once.  A key that no
    other particle of its list shares has exactly (number of greater keys)
    particles before it in any descending order, the reference's quicksort
    included, so those particles are placed by counting.  Only the order of
    equal keys depends on the quicksort's partitions: a list with equal keys
    is sorted again with the reference's algorithm (runner.c:110-174), in LDS,
-   and written whole: by a whole wave (sort_desc_wave), or by one lane when
-   a one-wave cell has more than MD_SORT_COOP_LISTS such lists.  For cells of
-   at most MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions, 13 * maxn list
-   words, 13 quicksort stacks, 13 pairs of partition tables. */
+   and written whole.  Cells of at most MD_SORT_ONEWAVE_MAX particles get one
+   wave, which sorts all such lists of its cell together (sort_lists_wave);
+   larger cells get 13 waves, wave l sorting list l (sort_desc_wave).  For
+   cells of at most MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions,
+   13 * maxn list words, 13 quicksort stacks, 13 pairs of partition tables
+   (the one-wave sort keeps its segment queues there). */
 #define MD_SORT_LDS_MAX 191
-#define MD_SORT_COOP_LISTS 7
+#define MD_SORT_ONEWAVE_MAX 64
 __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
                                                   const int *__restrict__ offset,
                                                   const unsigned int *__restrict__ sortflags,
@@ -567,11 +684,8 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
             tie[sid] = 1;
     }
     __syncthreads();
-    /* Lists with a repeated key: the reference's quicksort.  With 13 waves
-       per cell, wave l sorts list l cooperatively (sort_desc_wave).  With one
-       wave per cell (small cells), the wave sorts up to MD_SORT_COOP_LISTS
-       such lists one after another; above that, lane l sorts list l, the
-       13 serial sorts overlapping their LDS latencies. */
+    /* Lists with a repeated key: the reference's quicksort, by 13 waves one
+       list each, or by the cell's one wave all lists together. */
     const int wave = lane >> 6, nwave = (int)blockDim.x >> 6, wl = lane & 63;
     if (nwave > 1) {
         for (int l = wave; l < 13; l += nwave) {
@@ -586,20 +700,16 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
         unsigned int tm = 0;
         for (int l = 0; l < 13; l++) tm |= (unsigned int)(tie[l] != 0) << l;
         tm &= fl;
-        if (__builtin_popcount(tm) <= MD_SORT_COOP_LISTS) {
-            for (; tm; tm &= tm - 1) {
-                const int l = __builtin_ctz(tm);
-                unsigned int *L = lists + l * n;
-                int *st = stk + l * 2 * MD_SORT_STACK;
-                unsigned short *tb = tabs + (size_t)l * 2 * (maxn + 1);
-                sort_desc_wave(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
-                for (int i = wl; i < n; i += 64) out[l * n + i] = L[i];
+        if (tm) {
+            /* queues: the stacks' space, and the partition tables' space
+               after the first pair of tables */
+            unsigned short *tb = tabs + 2 * (maxn + 1);
+            sort_lists_wave(lists, n, tm, (unsigned int *)stk, 13 * 2 * MD_SORT_STACK, (unsigned int *)tb,
+                            12 * (maxn + 1), tabs, tabs + maxn + 1, flag, wl);
+            for (unsigned int t = tm; t; t &= t - 1) {
+                const int l = __builtin_ctz(t);
+                for (int i = wl; i < n; i += 64) out[l * n + i] = lists[l * n + i];
             }
-        } else if (lane < 13 && ((tm >> lane) & 1u)) {
-            unsigned int *L = lists + lane * n;
-            int *st = stk + lane * 2 * MD_SORT_STACK;
-            sort_desc_t(L, n, st, st + MD_SORT_STACK, flag);
-            for (int i = 0; i < n; i++) out[lane * n + i] = L[i];
         }
     }
 }
@@ -2425,7 +2535,7 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
                           sizeof(int) * 13 * 2 * MD_SORT_STACK + sizeof(unsigned short) * 13 * 2 * (maxn + 1);
         /* one wave per cell; for large cells (frequent repeated keys) one
            wave per direction */
-        const int bs = maxn > 48 ? 13 * 64 : 64;
+        const int bs = maxn > MD_SORT_ONEWAVE_MAX ? 13 * 64 : 64;
         hipLaunchKernelGGL(k_sort_rank, dim3(g->ncells), dim3(bs), sh, g->stream, P, g->d_count, g->d_offset,
                            g->d_sortflags, g->d_x, g->d_sortlist, g->d_keyof, g->d_queue + MD_NQUEUE, maxn);
     } else {

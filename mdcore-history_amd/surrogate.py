@@ -129,13 +129,14 @@ def drift_per_ns(etot, dt):
 
 
 def nve_drift(sysm, mlp=False, switch=None, equil=1000, steps=10000, dt=0.002, train_samples=1 << 20,
-              train_steps=10000, seed=7):
+              train_steps=10000, seed=7, sample_seed=5, train_seed=11):
     """Energy drift of `steps` device-resident NVE steps (engine_gpu_steps)
     after `equil` steps with the system's tables.  With mlp, the run after
     the equilibration evaluates the pairs with a network trained on the
     tables (half uniform-in-r samples, half the equilibrated configuration's
     pair distances), switched off to zero from `switch` to the cutoff if
-    given.  Total energy at step n: PE(x_n) + (KE(v_{n-1/2}) + KE(v_{n+1/2}))/2;
+    given (seed: initial weights; sample_seed: training samples;
+    train_seed: batch order).  Total energy at step n: PE(x_n) + (KE(v_{n-1/2}) + KE(v_{n+1/2}))/2;
     drift = least-squares slope per ns, relative to |<E>| (whole run and
     each half) and in kJ/mol."""
     import dataclasses
@@ -151,9 +152,9 @@ def nve_drift(sysm, mlp=False, switch=None, equil=1000, steps=10000, dt=0.002, t
         eng = systems.build_engine(s_eq, md.engine_flag_gpu)
         eng.setdt(dt)
         dom = pair_domains(eng, s_eq, switch=switch)
-        S = make_samples(eng, s_eq, train_samples, dom[0], dom[1], configs=[s_eq.x], seed=5)
+        S = make_samples(eng, s_eq, train_samples, dom[0], dom[1], configs=[s_eq.x], seed=sample_seed)
         eng.mlp_init(*dom, seed=seed)
-        eng.mlp_train(*S, steps=train_steps, batch_tiles=512, lr0=1e-2, lr1=1e-5, seed=11)
+        eng.mlp_train(*S, steps=train_steps, batch_tiles=512, lr0=1e-2, lr1=1e-5, seed=train_seed)
         eng.setflags(md.engine_flag_mlp, 0)
     ep, ek = eng.gpu_steps(steps)
     eng.close()

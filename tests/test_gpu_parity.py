@@ -42,6 +42,14 @@ SYSTEMS = {
     # simple cubic lattice, 64 atoms per cell: every direction's keys repeat
     # many times, so nearly every sort list goes through the tie sort
     "lattice_ties": lambda: lattice_ties(),
+    # perfect fcc crystal, 32 atoms per cell: the cell's one wave sorts
+    # nearly all 13 lists with repeated keys together
+    "fcc_lattice": lambda: fcc_ties(),
+    # the same with a 2 pm jitter: a few lists per cell keep repeated keys
+    "fcc_nearlattice": lambda: fcc_ties(jitter=0.002),
+    # 8 to 27 atoms per cell on a cubic lattice: lists of up to 15 atoms are
+    # a single selection-sort leaf, the longer ones partition first
+    "lattice_small_cells": lambda: lattice_ties(n=10, a=0.45),
     # pairs planted across cell faces: the reference window's unsigned break
     # wraps and skips whole pair tasks that hold in-range pairs
     "face_pairs": lambda: face_pairs(),
@@ -74,6 +82,22 @@ def lattice_ties(n=16, a=0.25, seed=17):
             (2, 2, md.LJ126(0.05, 1.0, 4e-6, 2.5e-3, 1e-3))]
     return systems.System("lattice", np.zeros(3), np.full(3, L), np.full(3, 1.0), 1.0, x, types,
                           np.array([0.0, 1.0, 2.0]), pots)
+
+
+def fcc_ties(n=8, a=0.565, jitter=0.0, seed=19):
+    """fcc crystal (argon's lattice constant), two species at random on the
+    sites so that the net forces do not cancel by symmetry."""
+    x = md.gen_fcc(n, n, n, a, a / 4)
+    L = n * a
+    if jitter > 0:
+        x = np.mod(x + md.gen_uniform(x.size, -0.5 * jitter, 0.5 * jitter, seed + 1).reshape(x.shape), L)
+    types = md.gen_types(len(x), 2, 1, seed)
+    # species 2 larger and deeper than argon: the pair forces on an atom
+    # cancel to about a fifth of their magnitude, not to a twenty-fifth
+    pots = [(1, 1, md.LJ126(0.275, 1.0, 9.7e-6, 6.2e-3, 1e-3)), (1, 2, md.LJ126(0.275, 1.0, 3e-5, 1.2e-2, 1e-3)),
+            (2, 2, md.LJ126(0.275, 1.0, 9e-5, 2.4e-2, 1e-3))]
+    return systems.System("fcc", np.zeros(3), np.full(3, L), np.full(3, 1.0), 1.0, x, types,
+                          np.array([0.0, 39.9, 39.9]), pots)
 
 
 def dense_big_cells(L=3.9, density=150.0, seed=3):
