@@ -135,8 +135,9 @@ int engine_gpu_steps(struct engine *e, int nsteps, double *epot, double *ekin);
 /* Paired visits (no reference counterpart): the full-shell table kernel
    visits the neighbour cells at opposite offsets +d and -d of an owner cell
    together, so that each lane walks its atom's window into both (long on one
-   side where it is short on the other).  engine_flag_unpaired visits them
-   one at a time. */
+   side where it is short on the other); with the reference window only
+   (engine_flag_projwindow passes visit one neighbour at a time).
+   engine_flag_unpaired visits them one at a time. */
 #define engine_flag_unpaired 2097152
 
 /* Species pairs: ti[p], tj[p] (ti <= tj) for p < returned count. */

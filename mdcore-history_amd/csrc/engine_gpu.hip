@@ -800,7 +800,7 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
                 adds its own contiguous run of results in order, so the
                 result stays deterministic.  Excluded pairs take the same
                 route, so a pair and its exclusion cancel exactly. */
-template <bool MLP, bool HALF, bool PAIRED = false>
+template <bool MLP, bool HALF, bool PAIRED = false, bool REFCNT = true>
 __device__ __forceinline__ void nonbond_body(
     unsigned char *smem, struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
     const int *__restrict__ nbr_off, const float4 *__restrict__ nbs, const double *__restrict__ origin,
@@ -1140,9 +1140,9 @@ __device__ __forceinline__ void nonbond_body(
                                 }
                                 float ea, fa, eb, fb;
                                 pot_eval32r2(ha, hb, ua, ub, rs_coef, r2a, r2b, ea, fa, eb, fb);
-                                lfx -= fa * dxa + fb * dxb;
-                                lfy -= fa * dya + fb * dyb;
-                                lfz -= fa * dza + fb * dzb;
+                                lfx = fmaf(-fb, dxb, fmaf(-fa, dxa, lfx));
+                                lfy = fmaf(-fb, dyb, fmaf(-fa, dya, lfy));
+                                lfz = fmaf(-fb, dzb, fmaf(-fa, dza, lfz));
                                 if constexpr (HALF) {
                                     /* the opposite force on the record's atom, into this
                                        lane's copy (lane mod 4) of the visit's array */
@@ -1165,14 +1165,14 @@ __device__ __forceinline__ void nonbond_body(
                                         le += ea;
                                         lv -= fa * r2a;
                                         lnp++;
-                                        lnref += (VK == 0 || !P.proj ||
+                                        if (REFCNT && P.proj) lnref += (VK == 0 ||
                                                   (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
                                     }
                                     if (cb) {
                                         le += eb;
                                         lv -= fb * r2b;
                                         lnp++;
-                                        lnref += (VK == 0 || !P.proj ||
+                                        if (REFCNT && P.proj) lnref += (VK == 0 ||
                                                   (live_ref && ((int)rec_key(wb) + en.z - myk) < dmd)) ? 1u : 0u;
                                     }
                                 }
@@ -1210,7 +1210,7 @@ __device__ __forceinline__ void nonbond_body(
                                     le += ea;
                                     lv -= fa * r2a;
                                     lnp++;
-                                    lnref += (VK == 0 || !P.proj ||
+                                    if (REFCNT && P.proj) lnref += (VK == 0 ||
                                               (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
                                 }
                             }
@@ -1246,7 +1246,7 @@ __device__ __forceinline__ void nonbond_body(
                                     le += ea;
                                     lv -= fa * r2a;
                                     lnp++;
-                                    lnref += (VK == 0 || !P.proj ||
+                                    if (REFCNT && P.proj) lnref += (VK == 0 ||
                                               (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
                                 }
                             }
@@ -1307,20 +1307,20 @@ __device__ __forceinline__ void nonbond_body(
                             }
                             float ea, fa, eb, fb;
                             pot_eval32r2(ha, hb, ua, ub, rs_coef, r2a, r2b, ea, fa, eb, fb);
-                            lfx -= fa * dxa + fb * dxb;
-                            lfy -= fa * dya + fb * dyb;
-                            lfz -= fa * dza + fb * dzb;
+                            lfx = fmaf(-fb, dxb, fmaf(-fa, dxa, lfx));
+                            lfy = fmaf(-fb, dyb, fmaf(-fa, dya, lfy));
+                            lfz = fmaf(-fb, dzb, fmaf(-fa, dza, lfz));
                             if (ua && ia) {
                                 le += ea;
                                 lv -= fa * r2a;
                                 lnp++;
-                                lnref += (!P.proj || (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd)) ? 1u : 0u;
+                                if (REFCNT && P.proj) lnref += (live_ref && ((int)rec_key(wa) + en.z - myk) < dmd) ? 1u : 0u;
                             }
                             if (ub && ib) {
                                 le += eb;
                                 lv -= fb * r2b;
                                 lnp++;
-                                lnref += (!P.proj || (live_ref && ((int)rec_key(wb) + en.z - myk) < dmd)) ? 1u : 0u;
+                                if (REFCNT && P.proj) lnref += (live_ref && ((int)rec_key(wb) + en.z - myk) < dmd) ? 1u : 0u;
                             }
                         }
                     }
@@ -1371,7 +1371,7 @@ __device__ __forceinline__ void nonbond_body(
                 epot += (double)le;
                 vir += (double)lv;
                 np += lnp;
-                nref += lnref;
+                nref += (REFCNT && P.proj) ? lnref : lnp;
             }
 
             /* Excluded pairs: subtract what the pair pass added
@@ -1572,17 +1572,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 }
 
 /* Table mode with paired visits (neighbour cells at opposite offsets
-   together), under the 4- and the 5-wave register budgets. */
+   together), under the 4- and the 5-wave register budgets.  Reference
+   window only: they do not count the pairs the reference window would
+   admit (with the reference window that count is the pair count). */
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbond_paired(NB_PARAMS) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const struct mlp_net N = {};
-    nonbond_body<false, false, true>(smem, NB_ARGS, N, nullptr);
+    nonbond_body<false, false, true, false>(smem, NB_ARGS, N, nullptr);
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_paired_wide(NB_PARAMS) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const struct mlp_net N = {};
-    nonbond_body<false, false, true>(smem, NB_ARGS, N, nullptr);
+    nonbond_body<false, false, true, false>(smem, NB_ARGS, N, nullptr);
 }
 
 /* Table mode for systems with many more cells than resident waves: the same
@@ -2445,11 +2447,13 @@ int gpu_upload(struct engine *e) {
 int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     /* kernel: 0 table, 1 surrogate, 2 table with the 5-wave budget, 3 / 4
        the half-shell table kernels (4 / 5 waves), 5 the half-shell
-       surrogate kernel */
+       surrogate kernel, 6 / 7 the paired table kernels (4 / 5 waves; not
+       for the projected window, whose pass also counts the reference
+       window's pairs) */
     const int wide = g->ncells >= 4 * 16 * g->ncu;
     const int kern = mlp      ? (g->mlp_half ? 5 : 1)
                      : g->half ? (wide ? 4 : 3)
-                     : g->paired ? (wide ? 7 : 6)
+                     : (g->paired && !g->P.proj) ? (wide ? 7 : 6)
                                  : (wide ? 2 : 0);
     struct nb_params P = g->P;
     int nb;
