@@ -453,6 +453,46 @@ __device__ __forceinline__ int2 part_wave(unsigned int *a, int lo, int hi, unsig
     return make_int2(i, j);
 }
 
+/* Two partition steps of part_wave at once, one per half wave: lanes 32h..
+   32h+31 partition a[lo..hi] (16 to 32 elements; lo > hi leaves the half
+   idle) with their half's tables ltab / rtab (33 entries each); lo, hi,
+   ltab and rtab are the lane's half's.  Returns the half's final (i, j). */
+__device__ __forceinline__ int2 part_wave2(unsigned int *a, int lo, int hi, unsigned short *ltab,
+                                           unsigned short *rtab, int lane) {
+    const unsigned long long half = (lane & 32) ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+    const unsigned long long below = ((1ull << lane) - 1ull) & half;
+    const int x = lo + (lane & 31);
+    const bool valid = x <= hi;
+    const unsigned int p = a[(lo + hi) >> 1] & 0xffffu;
+    const unsigned int w = valid ? a[x] : 0u;
+    const bool isl = valid && (w & 0xffffu) <= p, isr = valid && (w & 0xffffu) >= p;
+    const unsigned long long ml = __builtin_amdgcn_ballot_w64(isl) & half;
+    const unsigned long long mr = __builtin_amdgcn_ballot_w64(isr) & half;
+    const int nL = __builtin_popcountll(ml), nR = __builtin_popcountll(mr);
+    if (isl) ltab[__builtin_popcountll(ml & below) + 1] = (unsigned short)x;
+    if (isr) rtab[nR - __builtin_popcountll(mr & below)] = (unsigned short)x;
+    const int m = nL < nR ? nL : nR;
+    wave_lds_sync();
+    const int k = 1 + (lane & 31);
+    const int li = k <= m ? ltab[k] : 0, ri = k <= m ? rtab[k] : 0;
+    const int K = __builtin_popcountll(__builtin_amdgcn_ballot_w64(k <= m && li <= ri) & half);
+    if (k <= K && li < ri) {
+        const unsigned int va = a[li], vb = a[ri];
+        a[li] = vb;
+        a[ri] = va;
+    }
+    wave_lds_sync();
+    const int lK = ltab[K], rK = rtab[K];
+    int i = lK + 1, j = rK - 1;
+    if (i <= j) {
+        const int l1 = K + 1 <= nL ? (int)ltab[K + 1] : 1 << 30;
+        const int r1 = K + 1 <= nR ? (int)rtab[K + 1] : -(1 << 30);
+        i = l1 < rK ? l1 : rK;
+        j = r1 > lK ? r1 : lK;
+    }
+    return make_int2(i, j);
+}
+
 __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, unsigned short *ltab,
                                unsigned short *rtab, int *overflow, int lane) {
     int q = 0;
@@ -539,28 +579,38 @@ __device__ __forceinline__ unsigned int row_max_all(unsigned int v) {
    n <= 64 words at a + l * n), by one wave, with the same result.  A
    partition or a selection sort only touches its own segment, so the
    segments are taken in any order:
-   - segments of 16 or more elements are partitioned one at a time
-     (part_wave), their parts queued again;
+   - segments of 33 or more elements are partitioned by the whole wave
+     (part_wave), those of 16 to 32 two at a time, one per half wave
+     (part_wave2); their parts are queued again;
    - segments of 2..15 elements (the serial sort's selection-sort leaves)
      are sorted four at a time, one per row of 16 lanes: each step takes the
      first maximal key of the rest, the row max of
      key << 12 | (15 - row lane) << 8 | particle index
      (particle indices stay below 256), and swaps it with the step's lane,
      whose word every lane of the row gets from a second row max.
-   Segment queues: segq (segcap words) and leafq (leafcap words), lo | hi << 16
-   in list-array positions.  Reports a queue overflow or a runaway loop in
-   *overflow (the host turns it into a sort failure). */
-__device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigned int *segq, int segcap,
-                                unsigned int *leafq, int leafcap, unsigned short *ltab, unsigned short *rtab,
-                                int *overflow, int lane) {
-    int ns = 0, nl = 0;
+   Segment queues: segq (416 words: segments above 32 elements from the
+   front, 16 to 32 from the middle) and leafq (leafcap words), lo | hi << 16
+   in list-array positions.  Partition tables: tb, 136 entries.  Reports a
+   queue overflow or a runaway loop in *overflow (the host turns it into a
+   sort failure). */
+__device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigned int *segq, unsigned int *leafq,
+                                int leafcap, unsigned short *tb, unsigned int *__restrict__ out, int *overflow,
+                                int lane) {
+    constexpr int QHALF = 13 * MD_SORT_STACK;
+    int nbig = 0, nsmall = 0, nl = 0;
     bool bad = false;
     auto push = [&](int lo, int hi) {
         const unsigned int w = (unsigned int)lo | ((unsigned int)hi << 16);
-        if (hi - lo >= 15) {
-            if (ns < segcap) {
-                if (lane == 0) segq[ns] = w;
-                ns++;
+        if (hi - lo >= 32) {
+            if (nbig < QHALF) {
+                if (lane == 0) segq[nbig] = w;
+                nbig++;
+            } else
+                bad = true;
+        } else if (hi - lo >= 15) {
+            if (nsmall < QHALF) {
+                if (lane == 0) segq[QHALF + nsmall] = w;
+                nsmall++;
             } else
                 bad = true;
         } else if (hi > lo) {
@@ -571,6 +621,34 @@ __device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigne
                 bad = true;
         }
     };
+    /* Only the marked elements (keys that others of their list share) are
+       written from here on; rank placed the others.  A segment without a
+       marked element is dropped.  With one, that element's place is the
+       segment's start plus the number of greater keys in it (the others'
+       keys differ from its own).  With more, the segment is queued, and its
+       leaves write all of their elements in the end. */
+    auto place = [&](int lo, int hi) {
+        const int x = lo + lane;
+        const bool valid = x <= hi;
+        const unsigned int w = valid ? a[x] : 0u;
+        const unsigned long long mk = __builtin_amdgcn_ballot_w64(valid && (w >> 31) != 0u);
+        if (mk == 0ull) return;
+        if ((mk & (mk - 1ull)) == 0ull) {
+            const unsigned int wt = (unsigned int)__builtin_amdgcn_readlane((int)w, (int)__builtin_ctzll(mk));
+            const int g = __builtin_popcountll(__builtin_amdgcn_ballot_w64(valid && (w & 0xffffu) > (wt & 0xffffu)));
+            if (lane == 0) out[lo + g] = wt & 0x7fffffffu;
+            return;
+        }
+        push(lo, hi);
+    };
+    /* the elements a partition leaves between its two parts are in place */
+    auto settle = [&](int lo, int hi) {
+        const int x = lo + lane;
+        if (x <= hi) {
+            const unsigned int w = a[x];
+            if ((w >> 31) != 0u) out[x] = w & 0x7fffffffu;
+        }
+    };
     for (unsigned int t = tm; t; t &= t - 1) {
         const int l = __builtin_ctz(t);
         push(l * n, l * n + n - 1);
@@ -578,17 +656,46 @@ __device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigne
     wave_lds_sync();
     /* each partition splits a segment into two strictly smaller ones */
     int parts = 0;
-    while (ns > 0 && !bad) {
-        ns--;
-        const unsigned int sg = segq[ns];
-        const int lo = (int)(sg & 0xffffu), hi = (int)(sg >> 16);
-        if (++parts > 13 * n) {
-            bad = true;
-            break;
+    while ((nbig > 0 || nsmall > 0) && !bad) {
+        if (nbig > 0) {
+            nbig--;
+            const unsigned int sg = segq[nbig];
+            const int lo = (int)(sg & 0xffffu), hi = (int)(sg >> 16);
+            if (++parts > 13 * n) {
+                bad = true;
+                break;
+            }
+            const int2 ij = part_wave<1>(a, lo, hi, tb, tb + 68, lane);
+            wave_lds_sync();
+            place(lo, ij.y);
+            place(ij.x, hi);
+            settle(ij.y + 1, ij.x - 1);
+        } else {
+            const bool two = nsmall >= 2;
+            const unsigned int s0 = segq[QHALF + nsmall - 1];
+            const unsigned int s1 = two ? segq[QHALF + nsmall - 2] : 1u;
+            nsmall -= two ? 2 : 1;
+            parts += two ? 2 : 1;
+            if (parts > 13 * n) {
+                bad = true;
+                break;
+            }
+            const unsigned int sm = (lane & 32) ? s1 : s0;
+            const int lo = (int)(sm & 0xffffu), hi = (int)(sm >> 16);
+            unsigned short *lt = tb + ((lane & 32) ? 68 : 0);
+            const int2 ij = part_wave2(a, lo, hi, lt, lt + 34, lane);
+            const int i0 = __builtin_amdgcn_readlane(ij.x, 0), j0 = __builtin_amdgcn_readlane(ij.y, 0);
+            const int i1 = __builtin_amdgcn_readlane(ij.x, 32), j1 = __builtin_amdgcn_readlane(ij.y, 32);
+            wave_lds_sync();
+            place((int)(s0 & 0xffffu), j0);
+            place(i0, (int)(s0 >> 16));
+            settle(j0 + 1, i0 - 1);
+            if (two) {
+                place((int)(s1 & 0xffffu), j1);
+                place(i1, (int)(s1 >> 16));
+                settle(j1 + 1, i1 - 1);
+            }
         }
-        const int2 ij = part_wave<1>(a, lo, hi, ltab, rtab, lane);
-        push(lo, ij.y);
-        push(ij.x, hi);
         wave_lds_sync();
     }
     if (bad) {
@@ -607,7 +714,8 @@ __device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigne
         steps = max(steps, __builtin_amdgcn_readlane(len, 48));
         for (int i = 0; i < steps - 1; i++) {
             const bool in = lr >= i && lr < len;
-            const unsigned int pk = in ? ((w & 0xffffu) << 12) | ((unsigned int)(15 - lr) << 8) | (w >> 16) : 0u;
+            const unsigned int pk =
+                in ? ((w & 0xffffu) << 12) | ((unsigned int)(15 - lr) << 8) | ((w >> 16) & 0xffu) : 0u;
             const unsigned int mx = row_max_all(pk);
             const unsigned int wi = row_max_all(lr == i ? w : 0u);
             const int im = 15 - (int)((mx >> 8) & 15u);
@@ -619,8 +727,7 @@ __device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigne
                     w = wi;
             }
         }
-        if (lr < len) a[lo + lr] = w;
-        wave_lds_sync();
+        if (lr < len) out[lo + lr] = w & 0x7fffffffu;
     }
 }
 
@@ -635,7 +742,7 @@ __device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigne
    larger cells get 13 waves, wave l sorting list l (sort_desc_wave).  For
    cells of at most MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions,
    13 * maxn list words, 13 quicksort stacks, 13 pairs of partition tables
-   (the one-wave sort keeps its segment queues there). */
+   and 136 more entries (the one-wave sort keeps its segment queues there). */
 #define MD_SORT_LDS_MAX 191
 #define MD_SORT_ONEWAVE_MAX 64
 __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
@@ -678,10 +785,12 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
             gt += kj > key;
             eq += kj == key;
         }
-        if (eq == 1)
+        if (eq == 1) {
             out[sid * n + gt] = w;
-        else
+        } else {
             tie[sid] = 1;
+            lists[sid * n + i] = w | 0x80000000u; /* marked: shares its key */
+        }
     }
     __syncthreads();
     /* Lists with a repeated key: the reference's quicksort, by 13 waves one
@@ -694,22 +803,18 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
             int *st = stk + l * 2 * MD_SORT_STACK;
             unsigned short *tb = tabs + (size_t)l * 2 * (maxn + 1);
             sort_desc_wave(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
-            for (int i = wl; i < n; i += 64) out[l * n + i] = L[i];
+            for (int i = wl; i < n; i += 64) out[l * n + i] = L[i] & 0x7fffffffu;
         }
     } else {
         unsigned int tm = 0;
         for (int l = 0; l < 13; l++) tm |= (unsigned int)(tie[l] != 0) << l;
         tm &= fl;
         if (tm) {
-            /* queues: the stacks' space, and the partition tables' space
-               after the first pair of tables */
-            unsigned short *tb = tabs + 2 * (maxn + 1);
-            sort_lists_wave(lists, n, tm, (unsigned int *)stk, 13 * 2 * MD_SORT_STACK, (unsigned int *)tb,
-                            12 * (maxn + 1), tabs, tabs + maxn + 1, flag, wl);
-            for (unsigned int t = tm; t; t &= t - 1) {
-                const int l = __builtin_ctz(t);
-                for (int i = wl; i < n; i += 64) out[l * n + i] = lists[l * n + i];
-            }
+            /* segment queue: the stacks' space; partition tables and leaf
+               queue: the partition tables' space (26 (maxn + 1) + 136
+               entries) */
+            sort_lists_wave(lists, n, tm, (unsigned int *)stk, (unsigned int *)(tabs + 136), 13 * (maxn + 1), tabs,
+                            out, flag, wl);
         }
     }
 }
@@ -2536,7 +2641,7 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     } else if (g->maxcount <= MD_SORT_LDS_MAX) {
         const int maxn = g->maxcount > 0 ? g->maxcount : 1;
         const size_t sh = sizeof(double) * 3 * maxn + sizeof(unsigned int) * 13 * maxn +
-                          sizeof(int) * 13 * 2 * MD_SORT_STACK + sizeof(unsigned short) * 13 * 2 * (maxn + 1);
+                          sizeof(int) * 13 * 2 * MD_SORT_STACK + sizeof(unsigned short) * (13 * 2 * (maxn + 1) + 136);
         /* one wave per cell; for large cells (frequent repeated keys) one
            wave per direction */
         const int bs = maxn > MD_SORT_ONEWAVE_MAX ? 13 * 64 : 64;
