@@ -52,6 +52,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "gpu_state.h"
 #include "hostpack.h"
@@ -2779,6 +2780,12 @@ static int gpu_verlet(struct engine *e, struct gpu_state *g, int *sort) {
     return engine_err_ok;
 }
 
+static double gpu_wall(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 /* Counts of the layout the sort lists belong to. */
 static void gpu_sorted(struct engine *e, struct gpu_state *g) {
     for (int cid = 0; cid < e->s.nr_cells; cid++) g->h_sort_count[cid] = e->s.cells[cid].count;
@@ -2793,7 +2800,9 @@ extern "C" int engine_nonbond_gpu(struct engine *e) {
     if (g == NULL) return gerror_(engine_err_nogpu, "engine_gpu_load was not called");
     s = &e->s;
     HIPCHECK(hipSetDevice(g->device));
+    double tw = gpu_wall();
     if (gpu_upload(e) < 0) return engine_err_gpu;
+    e->timers[engine_timer_gpu_load] = gpu_wall() - tw;
     int sort = 1;
     if (gpu_verlet(e, g, &sort) < 0) return engine_err_range;
     HIPCHECK(hipMemsetAsync(g->d_queue + MD_NQUEUE, 0, sizeof(int), g->stream));
@@ -2812,7 +2821,9 @@ extern "C" int engine_nonbond_gpu(struct engine *e) {
         HIPCHECK(hipEventElapsedTime(&ms, g->ev[0], g->ev[3]));
         e->timers[engine_timer_gpu_kernels] = ms * 1e-3;
     }
+    tw = gpu_wall();
     space_add_forces_slots(s, g->h_offset, (const float *)g->h_fhi, (const float *)g->h_flo);
+    e->timers[engine_timer_gpu_unload] = gpu_wall() - tw;
     s->epot_nonbond += g->h_tot[0];
     s->epot_exclusion += g->h_tot[1];
     s->epot += g->h_tot[0] + g->h_tot[1];

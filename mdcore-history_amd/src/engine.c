@@ -321,59 +321,28 @@ int engine_getlocal(struct engine *e, double *x, int *cell, int N) {
    the kick-drift is done (the Verlet branch, 1961-1985). */
 int engine_advance(struct engine *e) {
     struct space *s = &e->s;
-    int cid, pid, k, delta[3];
+    int cid, pid, k;
     double epot = 0.0, dt = e->dt, w;
-    if (e->flags & engine_flag_verlet) {
-        /* no migration here: cells are independent (the energy sum stays in
-           cell order) */
-        for (cid = 0; cid < s->nr_real; cid++) epot += s->cells[s->cid_real[cid]].epot;
+    /* the energy sum in cell order */
+    for (cid = 0; cid < s->nr_real; cid++) epot += s->cells[s->cid_real[cid]].epot;
+    /* the kick-drift of every particle, each its own: in parallel */
 #pragma omp parallel for schedule(static) private(pid, k, w)
-        for (cid = 0; cid < s->nr_real; cid++) {
-            struct cell *c = &s->cells[s->cid_real[cid]];
-            for (pid = 0; pid < c->count; pid++) {
-                struct part *p = &c->parts[pid];
-                w = dt * e->types[p->type].imass;
-                for (k = 0; k < 3; k++) {
-                    p->v[k] += p->f[k] * w;
-                    p->x[k] += dt * p->v[k];
-                }
-            }
-        }
-        s->epot_nonbond += epot;
-        s->epot += epot;
-        return engine_err_ok;
-    }
     for (cid = 0; cid < s->nr_real; cid++) {
         struct cell *c = &s->cells[s->cid_real[cid]];
-        epot += c->epot;
-        pid = 0;
-        while (pid < c->count) {
+        for (pid = 0; pid < c->count; pid++) {
             struct part *p = &c->parts[pid];
             w = dt * e->types[p->type].imass;
             for (k = 0; k < 3; k++) {
                 p->v[k] += p->f[k] * w;
                 p->x[k] += dt * p->v[k];
-                delta[k] = (p->x[k] >= s->h[k]) - (p->x[k] < 0.0);
             }
-            if (delta[0] || delta[1] || delta[2]) {
-                struct cell *dest;
-                for (k = 0; k < 3; k++) p->x[k] -= delta[k] * s->h[k];
-                dest = &s->cells[space_cellid(s, (c->loc[0] + delta[0] + s->cdim[0]) % s->cdim[0],
-                                              (c->loc[1] + delta[1] + s->cdim[1]) % s->cdim[1],
-                                              (c->loc[2] + delta[2] + s->cdim[2]) % s->cdim[2])];
-                if (cell_add_incomming(dest, p) == NULL) return eerror_(engine_err_cell);
-                s->celllist[p->id] = dest;
-                c->count -= 1;
-                if (pid < c->count) {
-                    c->parts[pid] = c->parts[c->count];
-                    s->partlist[c->parts[pid].id] = &c->parts[pid];
-                }
-            } else
-                pid += 1;
         }
     }
-    for (cid = 0; cid < s->nr_marked; cid++)
-        if (cell_welcome(&s->cells[s->cid_marked[cid]], s->partlist) < 0) return eerror_(engine_err_cell);
+    /* the migration of the particles that left their cell: the same cell
+       contents and order as the reference's loop, which moves each particle
+       right after its kick-drift (a particle's move depends only on its own
+       position) */
+    if (!(e->flags & engine_flag_verlet) && space_shuffle_local(s) < 0) return eerror_(engine_err_cell);
     s->epot_nonbond += epot;
     s->epot += epot;
     return engine_err_ok;

@@ -374,12 +374,26 @@ int space_prepare(struct space *s) {
 
 /* Move particles that left their cell (x outside [0,h)) to the neighbour
    cell, filling the hole with the cell's last particle; newcomers are
-   appended in cell-id order of their origin, then particle order. */
+   appended in cell-id order of their origin, then particle order.  The
+   cells that hold a leaver are found in parallel; only they go through the
+   serial move, in cell order, so the result is that of one serial pass over
+   all cells.  The newcomers are then welcomed cell by cell in parallel. */
 int space_shuffle_local(struct space *s) {
-    int k, cid, pid, delta[3];
+    int k, cid, pid, delta[3], bad = 0;
     struct cell *c, *dest;
     struct part *p;
+    unsigned char *leaves = (unsigned char *)malloc(s->nr_real > 0 ? (size_t)s->nr_real : 1);
+    if (leaves == NULL) return serror_(space_err_malloc);
+#pragma omp parallel for schedule(static) private(pid, k)
     for (cid = 0; cid < s->nr_real; cid++) {
+        const struct cell *cc = &s->cells[s->cid_real[cid]];
+        unsigned char out = 0;
+        for (pid = 0; pid < cc->count && !out; pid++)
+            for (k = 0; k < 3; k++) out |= (cc->parts[pid].x[k] >= s->h[k]) | (cc->parts[pid].x[k] < 0.0);
+        leaves[cid] = out;
+    }
+    for (cid = 0; cid < s->nr_real; cid++) {
+        if (!leaves[cid]) continue;
         c = &s->cells[s->cid_real[cid]];
         pid = 0;
         while (pid < c->count) {
@@ -390,7 +404,10 @@ int space_shuffle_local(struct space *s) {
                 dest = &s->cells[space_cellid(s, (c->loc[0] + delta[0] + s->cdim[0]) % s->cdim[0],
                                               (c->loc[1] + delta[1] + s->cdim[1]) % s->cdim[1],
                                               (c->loc[2] + delta[2] + s->cdim[2]) % s->cdim[2])];
-                if (cell_add_incomming(dest, p) == NULL) return serror_(space_err_cell);
+                if (cell_add_incomming(dest, p) == NULL) {
+                    free(leaves);
+                    return serror_(space_err_cell);
+                }
                 s->celllist[p->id] = dest;
                 c->count -= 1;
                 if (pid < c->count) {
@@ -401,8 +418,13 @@ int space_shuffle_local(struct space *s) {
                 pid += 1;
         }
     }
+    free(leaves);
+    /* each cell appends its own newcomers and rewrites its own particles'
+       partlist entries */
+#pragma omp parallel for schedule(static) reduction(| : bad)
     for (cid = 0; cid < s->nr_marked; cid++)
-        if (cell_welcome(&s->cells[s->cid_marked[cid]], s->partlist) < 0) return serror_(space_err_cell);
+        bad |= cell_welcome(&s->cells[s->cid_marked[cid]], s->partlist) < 0;
+    if (bad) return serror_(space_err_cell);
     return space_err_ok;
 }
 
