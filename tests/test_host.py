@@ -163,3 +163,71 @@ def test_set_velocities_round_trip():
     with pytest.raises(ValueError):
         eng.set_velocities(v[:-1])
     eng.close()
+
+
+def _serial_advance(cells, xl, v, dt, cdim, h):
+    """The reference's advance on the host cells (engine.c:1945-2052), one
+    serial loop: per cell, each particle is kicked (no forces here) and
+    drifted, and one that left the cell moves at once to its neighbour's
+    incoming list, its slot taken by the cell's last particle; then each
+    cell appends its incoming particles in arrival order."""
+    cells = [list(c) for c in cells]
+    incoming = [[] for _ in cells]
+    xl = xl.copy()
+    for cid, parts in enumerate(cells):
+        loc = np.array(np.unravel_index(cid, cdim))
+        pid = 0
+        while pid < len(parts):
+            q = parts[pid]
+            xl[q] = xl[q] + dt * v[q]
+            delta = (xl[q] >= h).astype(int) - (xl[q] < 0.0).astype(int)
+            if delta.any():
+                xl[q] = xl[q] - delta * h
+                dest = np.ravel_multi_index(tuple((loc + delta) % cdim), cdim)
+                incoming[dest].append(q)
+                last = parts.pop()
+                if pid < len(parts):
+                    parts[pid] = last  # kicked and drifted when the loop reaches it
+            else:
+                pid += 1
+    return [c + inc for c, inc in zip(cells, incoming)], xl
+
+
+def test_parallel_advance_keeps_the_serial_order():
+    """engine_advance kick-drifts every particle in parallel and moves only
+    the particles that left their cell, cell by cell in order: the cells'
+    contents and order, and every position, equal one serial pass of the
+    reference's loop, with many particles crossing faces (a step of 0.25 of
+    the cell width per unit velocity)."""
+    import ctypes as C
+    sysm = systems.argon(6, jitter=0.05)
+    eng = systems.build_engine(sysm, md.engine_flag_gpu, start=False)
+    eng.shuffle()
+    N = sysm.N
+    v = md.gen_uniform(3 * N, -1.0, 1.0, 11).reshape(-1, 3)
+    eng.set_velocities(v)
+    cdim = tuple(int(n) for n in np.floor(sysm.dim / sysm.L))
+    h = sysm.dim / np.array(cdim)
+    ncells = int(np.prod(cdim))
+    counts, ids = eng.cells(ncells)
+    cells = np.split(ids, np.cumsum(counts)[:-1])
+
+    def local():
+        x = np.zeros(3 * N)
+        cell = np.zeros(N, dtype=np.int32)
+        assert eng.lib.engine_getlocal(eng.e, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                       cell.ctypes.data_as(C.POINTER(C.c_int)), N) == N
+        return x.reshape(-1, 3)
+
+    dt = 0.25 * float(h.min())
+    want_cells, want_x = _serial_advance(cells, local(), eng.velocities(), dt, cdim, h)
+    eng.setdt(dt)
+    eng.advance()
+    counts2, ids2 = eng.cells(ncells)
+    got = np.split(ids2, np.cumsum(counts2)[:-1])
+    moved = sum(len(a) != len(b) or not np.array_equal(a, b) for a, b in zip(cells, got))
+    assert moved > ncells // 2
+    for cid in range(ncells):
+        assert np.array_equal(got[cid], np.array(want_cells[cid], dtype=ids.dtype)), f"cell {cid}"
+    assert np.array_equal(local(), want_x)
+    eng.close()
