@@ -720,14 +720,20 @@ def eng_cells(sysm):
 
 
 def pair_kernel_name(sysm, md):
-    """The pair kernel engine_gpu_load picks: the half shell (with k_fold) for
-    engine_gpu_half_mean or more atoms per cell on average, otherwise the full
-    shell with paired visits; the 5-wave build from 64 cells per CU."""
+    """The pair kernel each upload picks: the half shell (with k_fold) for
+    engine_gpu_half_mean or more atoms per cell on average in fewer cells
+    than the device has CUs, otherwise the full shell with paired visits,
+    followed by k_fold when the owners are split (that occupancy in fewer
+    than 32 cells per CU); the 5-wave build from 64 cells per CU."""
     cells = eng_cells(sysm)
     wide = cells >= 64 * MI355X_CUS
-    if sysm.N >= md.engine_gpu_half_mean * cells:
+    dense = sysm.N >= md.engine_gpu_half_mean * cells
+    if dense and cells < MI355X_CUS:
         return ("k_nonbond_half_wide" if wide else "k_nonbond_half") + " + k_fold"
-    return "k_nonbond_paired_wide" if wide else "k_nonbond_paired"
+    name = "k_nonbond_paired_wide" if wide else "k_nonbond_paired"
+    if dense and cells < 32 * MI355X_CUS:
+        name += " + k_fold"
+    return name
 
 
 if __name__ == "__main__":

@@ -2526,13 +2526,17 @@ int gpu_upload(struct engine *e) {
     if (maxid + 1 > g->nids) return gerror_(engine_err_range, "particle id beyond the ids known at engine_gpu_load");
     g->nparts = n;
     g->maxcount = maxc;
-    g->half = (e->flags & engine_flag_halfshell) ? 1
+    g->paired = !(e->flags & engine_flag_unpaired);
+    /* default shell: the full shell, except for densely filled cells that
+       are too few to fill the device (fewer than its CUs), or that the
+       unpaired full-shell kernel would visit */
+    g->half = (e->flags & engine_flag_halfshell)   ? 1
               : (e->flags & engine_flag_fullshell) ? 0
-                                                   : (long long)n >= (long long)engine_gpu_half_mean * s->nr_cells;
+                                                   : (long long)n >= (long long)engine_gpu_half_mean * s->nr_cells &&
+                                                         (!g->paired || s->nr_cells < g->ncu);
     /* surrogate mode: the network evaluation dominates a visit, so the half
        shell's single evaluation per pair wins at any occupancy */
     g->mlp_half = !(e->flags & engine_flag_fullshell);
-    g->paired = !(e->flags & engine_flag_unpaired);
     g->P.nparts = n;
     g->P.nids = g->nids;
     g->P.maxn = maxc > 0 ? maxc : 1;

@@ -213,10 +213,14 @@ def test_repeatable(case):
 @pytest.mark.parametrize("name,expect", [("mixture_4k", "full"), ("water_1k", "half"), ("dense_bigcells", "half")])
 def test_shell_choice(name, expect):
     """Without a shell flag the engine takes the half shell for cells of
-    engine_gpu_half_mean atoms or more on average: its forces are bitwise
-    those of the forced shell (the two shells add in different orders)."""
+    engine_gpu_half_mean atoms or more on average when they are fewer than
+    the device's compute units (all three systems have fewer): its forces
+    are bitwise those of the forced shell (the two shells add in different
+    orders)."""
     sys = SYSTEMS[name]()
-    mean = len(sys.x) / np.prod(np.floor(sys.dim / sys.L))
+    ncells = np.prod(np.floor(sys.dim / sys.L))
+    mean = len(sys.x) / ncells
+    assert ncells < 256
     assert (mean >= md.engine_gpu_half_mean) == (expect == "half")
     out = {}
     for label, flags in (("auto", 0), ("full", md.engine_flag_fullshell), ("half", md.engine_flag_halfshell)):
