@@ -11,9 +11,10 @@
  *               the reference's descending quicksort (runner.c:110-174)
  *               replayed, so that the sorted lists, ties included, are the
  *               reference's (k_sort: the same per list in global memory, for
- *               cells of more than 191 particles).  Cells of more than 48
+ *               cells of more than 480 particles).  Cells of more than 64
  *               particles get a wave per direction and a wave-parallel
- *               restatement of that quicksort (sort_desc_wave);
+ *               restatement of that quicksort (sort_desc_wave); smaller
+ *               cells one wave that sorts all their lists (sort_lists_wave);
  *   k_nonbond   persistent kernel: each wave pulls an owner cell from one of
  *               8 device queues (one per XCD slab of cells), walks the cell's
  *               neighbour list (the reference's self/pair tasks that touch
@@ -494,6 +495,8 @@ __device__ __forceinline__ int2 part_wave2(unsigned int *a, int lo, int hi, unsi
     return make_int2(i, j);
 }
 
+/* R: up to 64 R elements per partition. */
+template <int R>
 __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, unsigned short *ltab,
                                unsigned short *rtab, int *overflow, int lane) {
     int q = 0;
@@ -531,7 +534,7 @@ __device__ void sort_desc_wave(unsigned int *a, int N, int *st_lo, int *st_hi, u
             wave_lds_sync();
             continue;
         }
-        const int2 ij = part_wave<3>(a, lo, hi, ltab, rtab, lane);
+        const int2 ij = part_wave<R>(a, lo, hi, ltab, rtab, lane);
         const int i = ij.x, j = ij.y;
         if (q + 2 >= MD_SORT_STACK) {
             if (lane == 0) *overflow = 1;
@@ -744,7 +747,7 @@ __device__ void sort_lists_wave(unsigned int *a, int n, unsigned int tm, unsigne
    cells of at most MD_SORT_LDS_MAX particles.  LDS: 3 * maxn positions,
    13 * maxn list words, 13 quicksort stacks, 13 pairs of partition tables
    and 136 more entries (the one-wave sort keeps its segment queues there). */
-#define MD_SORT_LDS_MAX 191
+#define MD_SORT_LDS_MAX 480 /* 63.5 KiB of LDS at 480 */
 #define MD_SORT_ONEWAVE_MAX 64
 __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const int *__restrict__ count,
                                                   const int *__restrict__ offset,
@@ -803,7 +806,10 @@ __global__ __launch_bounds__(1024) void k_sort_rank(struct nb_params P, const in
             unsigned int *L = lists + l * n;
             int *st = stk + l * 2 * MD_SORT_STACK;
             unsigned short *tb = tabs + (size_t)l * 2 * (maxn + 1);
-            sort_desc_wave(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
+            if (n <= 192)
+                sort_desc_wave<3>(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
+            else
+                sort_desc_wave<8>(L, n, st, st + MD_SORT_STACK, tb, tb + maxn + 1, flag, wl);
             for (int i = wl; i < n; i += 64) out[l * n + i] = L[i] & 0x7fffffffu;
         }
     } else {
