@@ -1940,6 +1940,7 @@ extern "C" int engine_gpu_finalize(struct engine *e) {
     (void)hipStreamSynchronize(g->stream);
     gpu_free_pass(g);
     gpu_mlp_free(g);
+    gpu_steps_free(g);
     (void)hipFree(g->d_ptab);
     (void)hipFree(g->d_coeffs);
     (void)hipFree(g->d_ptype);

@@ -138,6 +138,9 @@ struct gpu_state {
     float *d_sr, *d_sE, *d_sF;
     int *d_sp, *d_tiles;
     long long d_samples_cap;
+    /* engine_gpu_steps' device buffers (step_gpu.hip), kept between calls
+       and grown as needed; NULL until the first call */
+    void *steps;
 };
 
 /* Device view of the published network of g. */
@@ -156,6 +159,7 @@ static inline struct mlp_net gpu_mlp_net(const struct gpu_state *g) {
 }
 
 void gpu_mlp_free(struct gpu_state *g);
+void gpu_steps_free(struct gpu_state *g);
 
 /* engine_gpu.hip: pack the host cells and upload them (no kernels); launch
    one pass on the resident particles (mlp: surrogate mode; sort == 0 keeps

@@ -668,6 +668,65 @@ static int step_alloc(struct step_bufs *b, size_t N, int ncells, int ntypes, int
     return 0;
 }
 
+/* The step buffers of g for N particle slots, C cells, T types and S steps:
+   those of an earlier call when they are large enough (a call then spends
+   no time in device allocation), else new ones.  The rigid-group buffers
+   stay per call.  The arrival counts start at zero.  A call swaps layout
+   buffers between these and g's own, so it hands its set back at the end
+   (step_put). */
+struct step_cache {
+    struct step_bufs b;
+    size_t N;
+    int C, T, S;
+};
+
+static int step_get(struct gpu_state *g, struct step_bufs *b, size_t N, int C, int T, int S) {
+    struct step_cache *sc = (struct step_cache *)g->steps;
+    /* the particle and cell buffers trade places with g's, so they keep
+       exactly g's sizes */
+    if (sc == NULL || sc->N != N || sc->C != C || sc->T < T || sc->S < S) {
+        if (sc == NULL) {
+            if ((sc = (struct step_cache *)calloc(1, sizeof(*sc))) == NULL) return -1;
+            g->steps = sc;
+        } else {
+            /* no pending work may still use the old buffers */
+            if (hipStreamSynchronize(g->stream) != hipSuccess) return -1;
+            step_free(&sc->b);
+        }
+        sc->N = 0;
+        if (step_alloc(&sc->b, N, C, T, S) < 0) return -1;
+        sc->N = N;
+        sc->C = C;
+        sc->T = T;
+        sc->S = S;
+    }
+    *b = sc->b;
+    if (hipMemsetAsync(b->narr, 0, sizeof(int) * (size_t)C, g->stream) != hipSuccess) return -1;
+    return 0;
+}
+
+static void step_put(struct gpu_state *g, const struct step_bufs *b) {
+    struct step_cache *sc = (struct step_cache *)g->steps;
+    if (sc != NULL) sc->b = *b;
+}
+
+static void step_free_rigid(struct step_bufs *b) {
+    void *r[] = {b->rg_np, b->rg_nc, b->rg_parts, b->rg_ij, b->rg_d2, b->rg_a, b->sid, b->fcorr};
+    for (size_t k = 0; k < sizeof(r) / sizeof(r[0]); k++) (void)hipFree(r[k]);
+    b->rg_np = b->rg_nc = b->rg_parts = b->rg_ij = b->sid = NULL;
+    b->rg_d2 = b->fcorr = NULL;
+    b->rg_a = NULL;
+    b->nrig = 0;
+}
+
+void gpu_steps_free(struct gpu_state *g) {
+    struct step_cache *sc = (struct step_cache *)g->steps;
+    if (sc == NULL) return;
+    step_free(&sc->b);
+    free(sc);
+    g->steps = NULL;
+}
+
 /* Shuffle (es == NULL: no kick) or advance (kick-drift, then migration, the
    step's energies into es[0..3]): departures, the new layout, stayers and
    arrivals copied into it, then the layouts swapped.  Asynchronous; the
@@ -1037,7 +1096,7 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     }
     G.ncells = s->nr_cells;
     g->P.maxdx = NULL;
-    if (step_alloc(&b, (size_t)g->nparts_cap, g->ncells, g->max_type, nsteps) < 0)
+    if (step_get(g, &b, (size_t)g->nparts_cap, g->ncells, g->max_type, nsteps) < 0)
         return gerror_(engine_err_gpu, "step buffers");
     {
         double *mass = (double *)malloc(sizeof(double) * g->max_type);
@@ -1089,7 +1148,8 @@ extern "C" int engine_gpu_steps(struct engine *e, int nsteps, double *epot, doub
     if (rc == engine_err_ok && flag) rc = gerror_(engine_err_range, "sort stack overflow");
     if (rc == engine_err_ok) rc = step_download(e, g, &b, nsteps, epot, ekin, verlet, smaxdx);
     (void)hipStreamSynchronize(g->stream);
-    step_free(&b);
+    step_free_rigid(&b);
+    step_put(g, &b);
     /* without pseudo-Verlet the last sort lists belong to the layout before
        the last migration */
     if (!verlet || rc != engine_err_ok) g->sort_valid = 0;
