@@ -898,6 +898,7 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
 }
 
 #define MD_HCAP 32     /* table mode: hits buffered per lane before they are evaluated */
+#define MD_HCAP_WIDE 28 /* the same in the 5-wave builds: their LDS then allows 20 waves per CU */
 #define MD_SPLIT_ITEMS_PER_CU 32 /* pair-kernel work items per CU below which owners are split */
 #define MD_SPLIT_MAX 8           /* at most this many neighbour groups per owner cell */
 #define MD_HCAP_MLP 8  /* MLP mode: hits per lane per compaction round */
@@ -912,7 +913,7 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
                 adds its own contiguous run of results in order, so the
                 result stays deterministic.  Excluded pairs take the same
                 route, so a pair and its exclusion cancel exactly. */
-template <bool MLP, bool HALF, bool PAIRED = false, bool REFCNT = true>
+template <bool MLP, bool HALF, bool PAIRED = false, bool REFCNT = true, int HC = MD_HCAP>
 __device__ __forceinline__ void nonbond_body(
     unsigned char *smem, struct nb_params P, const int2 *__restrict__ cnt_off, const int4 *__restrict__ nbr,
     const int *__restrict__ nbr_off, const float4 *__restrict__ nbs, const double *__restrict__ origin,
@@ -922,7 +923,7 @@ __device__ __forceinline__ void nonbond_body(
     const float4 *__restrict__ ptab, const float4 *__restrict__ coeffs4, const int *__restrict__ excl_off,
     const int *__restrict__ excl_list, const int *__restrict__ index_of_id, const int *__restrict__ cell_of,
     struct nb_out O, int *__restrict__ queue, const struct mlp_net &N, const int *__restrict__ ptype) {
-    constexpr int HCAP = MLP ? MD_HCAP_MLP : MD_HCAP;
+    constexpr int HCAP = MLP ? MD_HCAP_MLP : HC;
     float4 *raw = (float4 *)smem;                                        /* [64]        */
     unsigned short *own_key = (unsigned short *)(smem + P.lds_own_key);  /* [13*64]     */
     int *kfl = (int *)(smem + P.lds_kfl);                                /* [26]        */
@@ -1696,7 +1697,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_paired_wide(NB_PARAMS) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const struct mlp_net N = {};
-    nonbond_body<false, false, true, false>(smem, NB_ARGS, N, nullptr);
+    nonbond_body<false, false, true, false, MD_HCAP_WIDE>(smem, NB_ARGS, N, nullptr);
 }
 
 /* Table mode for systems with many more cells than resident waves: the same
@@ -1706,7 +1707,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_wide(NB_PARAMS) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const struct mlp_net N = {};
-    nonbond_body<false, false>(smem, NB_ARGS, N, nullptr);
+    nonbond_body<false, false, false, true, MD_HCAP_WIDE>(smem, NB_ARGS, N, nullptr);
 }
 
 /* Surrogate mode (engine_flag_mlp). */
@@ -2586,7 +2587,7 @@ int gpu_launch(struct gpu_state *g, int timed, int mlp, int sort) {
     P.lds_stage = (int)lds;
     lds += sizeof(float4) * maxn * (kern >= 6 ? 2 : 1); /* paired visits stage two cells */
     P.lds_hit = (int)lds;
-    lds += sizeof(unsigned short) * 64 * (mlp ? MD_HCAP_MLP : MD_HCAP);
+    lds += sizeof(unsigned short) * 64 * (mlp ? MD_HCAP_MLP : (kern == 2 || kern == 7) ? MD_HCAP_WIDE : MD_HCAP);
     lds = (lds + 15) & ~(size_t)15;
     P.lds_dense = (int)lds;
     if (mlp) lds += 16 * 64 * MD_HCAP_MLP + 4 * 64 * MD_XCAP_MLP;
