@@ -870,27 +870,53 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane) {
 /* Evaluate the dense pair list [0, T) with the network, 16 pairs per tile;
    f = (dE/dr)/r and e of pair q go to dn_f[q], dn_e[q].  dE/dr is the
    network's own r-derivative, so the forces conserve the energies reported.
-   All 64 lanes must call this together (T is wave-uniform). */
-__device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const float *dn_r2, const int *dn_p,
-                                               float *dn_f, float *dn_e, int T, int lane) {
+   The list goes in chunks of 64 pairs: lane l first computes the inputs of
+   pair c + l (square root and four divisions, done once per pair instead of
+   in all four lanes that carry its column of a tile) and parks them in the
+   list's own slots (rn in dn_r2, irn in dn_f, dr_n/dr in dn_e; dr_n/dr
+   stays in its register), then the chunk's tiles run, each leaving y and
+   dy/dr of its pairs in dn_f and dn_e, and lane l turns those of pair c + l
+   into its energy and force.  dn_r2 does not keep r^2.  All 64 lanes must
+   call this together (T is wave-uniform). */
+__device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, float *dn_r2, const int *dn_p, float *dn_f,
+                                               float *dn_e, int T, int lane) {
     /* layer 2's operands stay in registers for the whole list: reloading
        them per tile cost 13 % of the 1M mixture's surrogate pass */
     mlp_bf16x8 w2r[8];
     float4 b2r[4];
     mlp_load_layer2(N, lane, w2r, b2r);
-    for (int base = 0; base < T; base += 16) {
-        const int q = base + (lane & 15);
-        const int qv = q < T ? q : T - 1;
-        const float r = sqrtf(dn_r2[qv]);
-        const int p = dn_p[qv];
-        float y, yd;
-        /* keep the network operands' loads inside the loop: hoisting them
-           out keeps ~100 registers live through the whole pair kernel */
-        asm volatile("" ::: "memory");
-        mlp_infer16_reg(N, w2r, b2r, r, p, lane, y, yd);
-        if (lane < 16 && q < T) {
+    for (int c = 0; c < T; c += 64) {
+        const int q = c + lane;
+        float r = 1.0f, drn = 0.0f;
+        if (q < T) {
+            r = sqrtf(dn_r2[q]);
+            float rn, irn, dirn;
+            mlp_inputs(N.sc[dn_p[q]], r, rn, irn, drn, dirn);
+            dn_r2[q] = rn;
+            dn_f[q] = irn;
+            dn_e[q] = dirn;
+        }
+        wave_lds_sync();
+        const int nt = min(4, (T - c + 15) >> 4);
+        for (int t = 0; t < nt; t++) {
+            const int qt = c + 16 * t + (lane & 15);
+            const int qv = qt < T ? qt : T - 1;
+            const float drt = __shfl(drn, qv - c);
+            /* keep the network operands' loads inside the loop: hoisting them
+               out keeps ~100 registers live through the whole pair kernel */
+            asm volatile("" ::: "memory");
+            float y, yd;
+            mlp_infer16_reg(N, w2r, b2r, dn_r2[qv], dn_f[qv], drt, dn_e[qv], dn_p[qv], lane, y, yd);
+            if (lane < 16 && qt < T) {
+                dn_f[qt] = y;
+                dn_e[qt] = yd;
+            }
+        }
+        wave_lds_sync();
+        if (q < T) {
+            const int p = dn_p[q];
             float e, dedr;
-            mlp_energy_force(N.sc[p], N.shift[p], r, y, yd, e, dedr);
+            mlp_energy_force(N.sc[p], N.shift[p], r, dn_f[q], dn_e[q], e, dedr);
             dn_f[q] = dedr / r;
             dn_e[q] = e;
         }

@@ -106,29 +106,28 @@ __device__ __forceinline__ void mlp_inputs(const float4 sc, float r, float &rn, 
 }
 
 /* Forward pass of a 16-pair tile with its r-derivative.  Every lane passes
-   the (r, p) of pair l % 16 (lanes l, l+16, l+32, l+48 pass the same pair).
-   Results: y and yd = dy/dr of that pair in every lane.  With KEEP, the
-   lane's z1/t1/z2/t2 and the hi halves of the bf16 h1/g1 operands are kept
-   for the backward pass (its dW2 product takes the hi halves); without it (inference) each 16-unit block goes layer 2 -> SiLU ->
-   head as soon as its MFMAs are done, so few registers stay live.  With
+   the inputs (mlp_inputs) and p of pair l % 16 (lanes l, l+16, l+32, l+48
+   pass the same pair).  Results: y and yd = dy/dr of that pair in every
+   lane.  With KEEP, the lane's z1/t1/z2/t2 and the hi halves of the bf16
+   h1/g1 operands are kept for the backward pass (its dW2 product takes the
+   hi halves); without it (inference) each 16-unit block goes layer 2 ->
+   SiLU -> head as soon as its MFMAs are done, so few registers stay live.  With
    REG, layer 2's A operands and biases come from registers the caller
    loaded once (mlp_load_layer2) instead of memory.  All variants do the
    same arithmetic in the same order, so they give the same bits. */
 template <bool KEEP, bool REG>
-__device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, const mlp_bf16x8 *w2r, const float4 *b2r, float r,
-                                                int p, int lane, float *z1, float *t1, mlp_bf16x8 *h1b,
-                                                mlp_bf16x8 *g1b, float *z2, float *t2, float &y, float &yd) {
-    const float4 sc = N.sc[p];
-    float rn, irn, drn, dirn;
-    mlp_inputs(sc, r, rn, irn, drn, dirn);
+__device__ __forceinline__ void mlp_forward16_in(const mlp_net &N, const mlp_bf16x8 *w2r, const float4 *b2r, float rn,
+                                                 float irn, float drn, float dirn, int p, int lane, float *z1,
+                                                 float *t1, mlp_bf16x8 *h1b, mlp_bf16x8 *g1b, float *z2, float *t2,
+                                                 float &y, float &yd) {
     const float *b1e = N.b1e + (size_t)p * MLP_H;
     mlp_bf16x8 hb[2], hl[2], gb[2], gl[2];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int u = mlp_unit(j, lane);
         const float2 w = *(const float2 *)(N.w1 + 2 * u);
-        const float z = w.x * rn + w.y * irn + b1e[u];
-        const float t = w.x * drn + w.y * dirn;
+        const float z = fmaf(w.y, irn, fmaf(w.x, rn, b1e[u]));
+        const float t = fmaf(w.y, dirn, w.x * drn);
         if (KEEP) {
             z1[j] = z;
             t1[j] = t;
@@ -157,7 +156,10 @@ __device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, const mlp_bf16
     for (int mb = 0; mb < 4; mb++) {
         const mlp_bf16x8 a0 = REG ? w2r[mb * 2 + 0] : N.w2f[(mb * 2 + 0) * 64 + lane];
         const mlp_bf16x8 a1 = REG ? w2r[mb * 2 + 1] : N.w2f[(mb * 2 + 1) * 64 + lane];
-        mlp_f4 acc = {0.f, 0.f, 0.f, 0.f}, act = {0.f, 0.f, 0.f, 0.f};
+        const int u0 = 16 * mb + 4 * (lane >> 4);
+        const float4 bb = REG ? b2r[mb] : *(const float4 *)(N.b2 + u0);
+        /* the biases start the accumulation */
+        mlp_f4 acc = {bb.x, bb.y, bb.z, bb.w}, act = {0.f, 0.f, 0.f, 0.f};
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, hb[0], acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, hb[1], acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, hl[0], acc, 0, 0, 0);
@@ -166,10 +168,8 @@ __device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, const mlp_bf16
         act = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, gb[1], act, 0, 0, 0);
         act = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, gl[0], act, 0, 0, 0);
         act = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, gl[1], act, 0, 0, 0);
-        const int u0 = 16 * mb + 4 * (lane >> 4);
-        const float4 bb = REG ? b2r[mb] : *(const float4 *)(N.b2 + u0);
         const float4 wh = *(const float4 *)(w3 + u0);
-        const float zz[4] = {acc[0] + bb.x, acc[1] + bb.y, acc[2] + bb.z, acc[3] + bb.w};
+        const float zz[4] = {acc[0], acc[1], acc[2], acc[3]};
         const float ww[4] = {wh.x, wh.y, wh.z, wh.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -189,6 +189,16 @@ __device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, const mlp_bf16
     s1 += __shfl_xor(s1, 32);
     y = s0 + N.b3[p];
     yd = s1;
+}
+
+/* The same from the pair distance: the inputs of pair p at r first. */
+template <bool KEEP, bool REG>
+__device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, const mlp_bf16x8 *w2r, const float4 *b2r, float r,
+                                                int p, int lane, float *z1, float *t1, mlp_bf16x8 *h1b,
+                                                mlp_bf16x8 *g1b, float *z2, float *t2, float &y, float &yd) {
+    float rn, irn, drn, dirn;
+    mlp_inputs(N.sc[p], r, rn, irn, drn, dirn);
+    mlp_forward16_in<KEEP, REG>(N, w2r, b2r, rn, irn, drn, dirn, p, lane, z1, t1, h1b, g1b, z2, t2, y, yd);
 }
 
 /* Training forward: keeps the activations of this lane's 16 units. */
@@ -214,11 +224,13 @@ __device__ __forceinline__ void mlp_load_layer2(const mlp_net &N, int lane, mlp_
     for (int mb = 0; mb < 4; mb++) b2r[mb] = *(const float4 *)(N.b2 + 16 * mb + 4 * (lane >> 4));
 }
 
-/* Inference forward with layer 2's operands from mlp_load_layer2. */
-__device__ __forceinline__ void mlp_infer16_reg(const mlp_net &N, const mlp_bf16x8 w2r[8], const float4 b2r[4], float r,
-                                                int p, int lane, float &y, float &yd) {
-    mlp_forward16_t<false, true>(N, w2r, b2r, r, p, lane, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, y,
-                                 yd);
+/* Inference forward with layer 2's operands from mlp_load_layer2 and the
+   pair's inputs (mlp_inputs) computed by the caller. */
+__device__ __forceinline__ void mlp_infer16_reg(const mlp_net &N, const mlp_bf16x8 w2r[8], const float4 b2r[4],
+                                                float rn, float irn, float drn, float dirn, int p, int lane, float &y,
+                                                float &yd) {
+    mlp_forward16_in<false, true>(N, w2r, b2r, rn, irn, drn, dirn, p, lane, nullptr, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, y, yd);
 }
 
 /* Energy and dE/dr of one pair from the network outputs y, yd (see the
