@@ -873,8 +873,8 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane) {
    The list goes in chunks of 64 pairs: lane l first computes the inputs of
    pair c + l (square root and four divisions, done once per pair instead of
    in all four lanes that carry its column of a tile) and parks them in the
-   list's own slots (rn in dn_r2, irn in dn_f, dr_n/dr in dn_e; dr_n/dr
-   stays in its register), then the chunk's tiles run, each leaving y and
+   list's own slots (rn in dn_r2, irn in dn_f, d irn / dr in dn_e; d rn / dr
+   stays in lane l's register), then the chunk's tiles run, each leaving y and
    dy/dr of its pairs in dn_f and dn_e, and lane l turns those of pair c + l
    into its energy and force.  dn_r2 does not keep r^2.  All 64 lanes must
    call this together (T is wave-uniform). */
