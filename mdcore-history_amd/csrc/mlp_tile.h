@@ -108,7 +108,8 @@ __device__ __forceinline__ void mlp_inputs(const float4 sc, float r, float &rn, 
 /* Forward pass of a 16-pair tile with its r-derivative.  Every lane passes
    the inputs (mlp_inputs) and p of pair l % 16 (lanes l, l+16, l+32, l+48
    pass the same pair).  Results: y and yd = dy/dr of that pair in every
-   lane.  With KEEP, the lane's z1/t1/z2/t2 and the hi halves of the bf16
+   lane.  w1 points at layer 1's weights [64][2] (N.w1 or a copy).  With
+   KEEP, the lane's z1/t1/z2/t2 and the hi halves of the bf16
    h1/g1 operands are kept for the backward pass (its dW2 product takes the
    hi halves); without it (inference) each 16-unit block goes layer 2 ->
    SiLU -> head as soon as its MFMAs are done, so few registers stay live.  With
@@ -116,8 +117,9 @@ __device__ __forceinline__ void mlp_inputs(const float4 sc, float r, float &rn, 
    loaded once (mlp_load_layer2) instead of memory.  All variants do the
    same arithmetic in the same order, so they give the same bits. */
 template <bool KEEP, bool REG>
-__device__ __forceinline__ void mlp_forward16_in(const mlp_net &N, const mlp_bf16x8 *w2r, const float4 *b2r, float rn,
-                                                 float irn, float drn, float dirn, int p, int lane, float *z1,
+__device__ __forceinline__ void mlp_forward16_in(const mlp_net &N, const float *w1, const mlp_bf16x8 *w2r,
+                                                 const float4 *b2r, float rn, float irn, float drn, float dirn, int p,
+                                                 int lane, float *z1,
                                                  float *t1, mlp_bf16x8 *h1b, mlp_bf16x8 *g1b, float *z2, float *t2,
                                                  float &y, float &yd) {
     const float *b1e = N.b1e + (size_t)p * MLP_H;
@@ -125,7 +127,7 @@ __device__ __forceinline__ void mlp_forward16_in(const mlp_net &N, const mlp_bf1
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int u = mlp_unit(j, lane);
-        const float2 w = *(const float2 *)(N.w1 + 2 * u);
+        const float2 w = *(const float2 *)(w1 + 2 * u);
         const float z = fmaf(w.y, irn, fmaf(w.x, rn, b1e[u]));
         const float t = fmaf(w.y, dirn, w.x * drn);
         if (KEEP) {
@@ -198,7 +200,7 @@ __device__ __forceinline__ void mlp_forward16_t(const mlp_net &N, const mlp_bf16
                                                 mlp_bf16x8 *g1b, float *z2, float *t2, float &y, float &yd) {
     float rn, irn, drn, dirn;
     mlp_inputs(N.sc[p], r, rn, irn, drn, dirn);
-    mlp_forward16_in<KEEP, REG>(N, w2r, b2r, rn, irn, drn, dirn, p, lane, z1, t1, h1b, g1b, z2, t2, y, yd);
+    mlp_forward16_in<KEEP, REG>(N, N.w1, w2r, b2r, rn, irn, drn, dirn, p, lane, z1, t1, h1b, g1b, z2, t2, y, yd);
 }
 
 /* Training forward: keeps the activations of this lane's 16 units. */
@@ -224,12 +226,13 @@ __device__ __forceinline__ void mlp_load_layer2(const mlp_net &N, int lane, mlp_
     for (int mb = 0; mb < 4; mb++) b2r[mb] = *(const float4 *)(N.b2 + 16 * mb + 4 * (lane >> 4));
 }
 
-/* Inference forward with layer 2's operands from mlp_load_layer2 and the
-   pair's inputs (mlp_inputs) computed by the caller. */
-__device__ __forceinline__ void mlp_infer16_reg(const mlp_net &N, const mlp_bf16x8 w2r[8], const float4 b2r[4],
-                                                float rn, float irn, float drn, float dirn, int p, int lane, float &y,
-                                                float &yd) {
-    mlp_forward16_in<false, true>(N, w2r, b2r, rn, irn, drn, dirn, p, lane, nullptr, nullptr, nullptr, nullptr,
+/* Inference forward with layer 2's operands from mlp_load_layer2, the
+   pair's inputs (mlp_inputs) computed by the caller and layer 1's weights
+   w1 (N.w1 or a copy of it, e.g. in LDS). */
+__device__ __forceinline__ void mlp_infer16_reg(const mlp_net &N, const float *w1, const mlp_bf16x8 w2r[8],
+                                                const float4 b2r[4], float rn, float irn, float drn, float dirn, int p,
+                                                int lane, float &y, float &yd) {
+    mlp_forward16_in<false, true>(N, w1, w2r, b2r, rn, irn, drn, dirn, p, lane, nullptr, nullptr, nullptr, nullptr,
                                   nullptr, nullptr, y, yd);
 }
 
