@@ -1337,8 +1337,7 @@ __device__ __forceinline__ void nonbond_body(
                                 if (__float_as_int(ha.w) == -1) continue;
                                 const float dxa = xi.x - pa.x, dya = xi.y - pa.y, dza = xi.z - pa.z;
                                 const float r2a = dxa * dxa + dya * dya + dza * dza;
-                                const float rlo = N.sc[ptype[ti * mt + rec_type(wa, P.tshift)]].x;
-                                if (r2a >= rlo * rlo) {
+                                if (r2a >= N.rlo2[ti * mt + rec_type(wa, P.tshift)]) {
                                     hitl[nd * 64 + lane] = (unsigned short)ta;
                                     nd++;
                                     continue;
@@ -1597,8 +1596,7 @@ __device__ __forceinline__ void nonbond_body(
                                            P.h) <= P.cutoff2;
                         }
                         if (!in) continue;
-                        const float rlo = N.sc[ptype[ti * mt + tb]].x;
-                        if (r2 >= rlo * rlo) {
+                        if (r2 >= N.rlo2[ti * mt + tb]) {
                             exb[nd * 64 + lane] = b;
                             nd++;
                             continue;

@@ -454,6 +454,8 @@ void gpu_mlp_free(struct gpu_state *g) {
     (void)hipFree(g->d_w2f);
     (void)hipFree(g->d_w2b);
     (void)hipFree(g->d_msc);
+    (void)hipFree(g->d_mrlo2);
+    g->d_mrlo2 = NULL;
     (void)hipFree(g->d_msh);
     (void)hipFree(g->d_mshy);
     free(g->h_msc);
@@ -631,6 +633,32 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
         free(sc);
         gpu_mlp_free(g);
         return gerror_(engine_err_gpu, "MLP allocation failed");
+    }
+    {
+        /* r_lo^2 by type pair, so that the pair kernels' split between the
+           tables and the network is one load; the same FP32 product as
+           r_lo * r_lo on the device */
+        const int mt = g->max_type;
+        float *rl2 = (float *)malloc(sizeof(float) * mt * mt);
+        if (rl2 == NULL) {
+            free(theta);
+            free(sc);
+            gpu_mlp_free(g);
+            return gerror_(engine_err_malloc, "malloc failed");
+        }
+        for (k = 0; k < mt * mt; k++) {
+            const int p = g->h_tabof[k];
+            rl2[k] = p >= 0 ? sc[p].x * sc[p].x : 0.0f;
+        }
+        const int bad = hipMalloc(&g->d_mrlo2, sizeof(float) * mt * mt) != hipSuccess ||
+                        hipMemcpy(g->d_mrlo2, rl2, sizeof(float) * mt * mt, hipMemcpyHostToDevice) != hipSuccess;
+        free(rl2);
+        if (bad) {
+            free(theta);
+            free(sc);
+            gpu_mlp_free(g);
+            return gerror_(engine_err_gpu, "MLP allocation failed");
+        }
     }
     free(theta);
     g->h_msc = sc;
