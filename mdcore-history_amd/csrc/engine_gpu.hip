@@ -1333,15 +1333,17 @@ __device__ __forceinline__ void nonbond_body(
                                 const int ta = hitl[s * 64 + lane];
                                 const float4 pa = stage[ta];
                                 const unsigned int wa = __float_as_uint(pa.w);
-                                const float4 ha = ptab[ti * mt + rec_type(wa, P.tshift)];
-                                if (__float_as_int(ha.w) == -1) continue;
                                 const float dxa = xi.x - pa.x, dya = xi.y - pa.y, dza = xi.z - pa.z;
                                 const float r2a = dxa * dxa + dya * dya + dza * dza;
+                                /* r_lo^2 is +inf for a type pair without potential, so
+                                   only the few hits below r_lo load the table header */
                                 if (r2a >= N.rlo2[ti * mt + rec_type(wa, P.tshift)]) {
                                     hitl[nd * 64 + lane] = (unsigned short)ta;
                                     nd++;
                                     continue;
                                 }
+                                const float4 ha = ptab[ti * mt + rec_type(wa, P.tshift)];
+                                if (__float_as_int(ha.w) == -1) continue;
                                 float ea, fa;
                                 pot_eval32(ha, coeffs4, r2a, ea, fa);
                                 lfx -= fa * dxa;

@@ -134,7 +134,7 @@ struct gpu_state {
     float *d_tpart, *d_err; /* per-tile gradient rows; bad-pair flag */
     mlp_bf16x8 *d_w2f, *d_w2b;
     float4 *d_msc, *h_msc;  /* per pair (r_lo, r_hi, sE, r_switch) */
-    float *d_mrlo2;         /* [mt*mt] per type pair: r_lo^2 of its species pair (FP32 product) */
+    float *d_mrlo2;         /* [mt*mt] per type pair: r_lo^2 of its species pair (FP32 product), +inf without one */
     float *d_msh, *d_mshy;  /* per pair energy shift; y at r_hi (scratch of the shift) */
     float *d_sr, *d_sE, *d_sF;
     int *d_sp, *d_tiles;

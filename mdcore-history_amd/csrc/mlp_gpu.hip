@@ -648,7 +648,7 @@ extern "C" int engine_mlp_init(struct engine *e, const double *r_lo, const doubl
         }
         for (k = 0; k < mt * mt; k++) {
             const int p = g->h_tabof[k];
-            rl2[k] = p >= 0 ? sc[p].x * sc[p].x : 0.0f;
+            rl2[k] = p >= 0 ? sc[p].x * sc[p].x : INFINITY;
         }
         const int bad = hipMalloc(&g->d_mrlo2, sizeof(float) * mt * mt) != hipSuccess ||
                         hipMemcpy(g->d_mrlo2, rl2, sizeof(float) * mt * mt, hipMemcpyHostToDevice) != hipSuccess;

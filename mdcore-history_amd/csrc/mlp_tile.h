@@ -74,7 +74,7 @@ struct mlp_net {
     const float *b3;         /* [np]                                        */
     const float4 *sc;        /* [np] = (r_lo, r_hi, sE, r_sw)               */
     const float *shift;      /* [np]                                        */
-    const float *rlo2;       /* [mt*mt] per type pair: sc[p].x^2, p its pair */
+    const float *rlo2;       /* [mt*mt] per type pair: sc[p].x^2 (p its pair) or +inf */
 };
 
 __device__ __forceinline__ int mlp_unit(int j, int lane) { return 16 * (j >> 2) + 4 * (lane >> 4) + (j & 3); }
