@@ -874,9 +874,9 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane) {
    pair c + l (square root and four divisions, done once per pair instead of
    in all four lanes that carry its column of a tile) and parks them in the
    list's own slots (rn in dn_r2, irn in dn_f, d irn / dr in dn_e; d rn / dr
-   stays in lane l's register), then the chunk's tiles run, each leaving y and
-   dy/dr of its pairs in dn_f and dn_e, and lane l turns those of pair c + l
-   into its energy and force.  dn_r2 does not keep r^2.  w1s: the wave's
+   stays in lane l's register), then the chunk's tiles run, each leaving y
+   (without the output bias) and dy/dr of its pairs in dn_f and dn_e, and
+   lane l turns those of pair c + l into its energy and force.  dn_r2 does not keep r^2.  w1s: the wave's
    LDS copy of layer 1's weights (read there, every tile's layer-1 loads
    leave the vector memory path: 40.9 -> 39.9 ms on the 1M mixture).  All
    64 lanes must call this together (T is wave-uniform). */
@@ -918,7 +918,7 @@ __device__ __forceinline__ void mlp_dense_eval(const struct mlp_net &N, const fl
         if (q < T) {
             const int p = dn_p[q];
             float e, dedr;
-            mlp_energy_force(N.sc[p], N.shift[p], r, dn_f[q], dn_e[q], e, dedr);
+            mlp_energy_force(N.sc[p], N.shift[p], r, dn_f[q] + N.b3[p], dn_e[q], e, dedr);
             dn_f[q] = dedr / r;
             dn_e[q] = e;
         }

@@ -190,7 +190,8 @@ __device__ __forceinline__ void mlp_forward16_in(const mlp_net &N, const float *
     s1 += __shfl_xor(s1, 16);
     s0 += __shfl_xor(s0, 32);
     s1 += __shfl_xor(s1, 32);
-    y = s0 + N.b3[p];
+    /* REG (the pair pass): the caller adds b3[p], once per pair */
+    y = REG ? s0 : s0 + N.b3[p];
     yd = s1;
 }
 
@@ -229,7 +230,8 @@ __device__ __forceinline__ void mlp_load_layer2(const mlp_net &N, int lane, mlp_
 
 /* Inference forward with layer 2's operands from mlp_load_layer2, the
    pair's inputs (mlp_inputs) computed by the caller and layer 1's weights
-   w1 (N.w1 or a copy of it, e.g. in LDS). */
+   w1 (N.w1 or a copy of it, e.g. in LDS).  y comes without the output bias:
+   y + N.b3[p] is the network's output. */
 __device__ __forceinline__ void mlp_infer16_reg(const mlp_net &N, const float *w1, const mlp_bf16x8 w2r[8],
                                                 const float4 b2r[4], float rn, float irn, float drn, float dirn, int p,
                                                 int lane, float &y, float &yd) {
